@@ -1,0 +1,116 @@
+/*
+ * tree_layouts.h -- C ABI of the MI355X (gfx950) Tree-Layouts library.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are
+ * hipMalloc'ed by the caller; `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream).  When `ms` is non-NULL the call waits for its
+ * work and stores the elapsed GPU time in milliseconds (the reference's
+ * functions are synchronous and return a time); when `ms` is NULL the call
+ * only enqueues work on `stream`.
+ *
+ * Query answers are always uint64: the slot of the query key in the layout,
+ * or n when the key is absent (the reference's own miss value,
+ * cpu/src/query-bst.h:42, cpu/src/query-btree.h:53).
+ */
+#ifndef TREE_LAYOUTS_H
+#define TREE_LAYOUTS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TL_OK = 0,
+    TL_EINVAL = 1,      /* bad size / width / pointer combination */
+    TL_ENOMEM = 2,      /* device scratch could not be allocated */
+    TL_EHIP = 3,        /* a HIP call failed; see tl_last_error() */
+    TL_ETIMEOUT = 4,    /* a bounded in-kernel wait gave up (never expected) */
+    TL_EUNSUPPORTED = 5 /* combination this build does not implement */
+} tl_status;
+
+typedef enum { TL_U32 = 0, TL_U64 = 1, TL_I32 = 2, TL_I64 = 3 } tl_dtype;
+
+/* Layout-compatible with the reference's struct vEB_table
+ * (gpu/src/query-mveb.cuh:27-31): {u64 L; u64 R; u32 D;}, 24-byte stride. */
+typedef struct {
+    uint64_t L;
+    uint64_t R;
+    uint32_t D;
+} tl_veb_table;
+
+const char *tl_last_error(void);
+const char *tl_version(void);
+
+/* ---- in-place permutations of a sorted array -------------------------- */
+
+/* Level-order BST (Eytzinger).  Replaces
+ * timePermuteBST<T>(T *dev_A, uint64_t n)  (gpu/src/bst-hybrid.cuh:120). */
+tl_status tl_permute_bst(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+
+/* Level-order B-tree with b keys per node.  Replaces
+ * timePermuteBtree<T>(T *dev_A, uint64_t n, uint64_t b)
+ * (gpu/src/btree-cycles.cuh:77). */
+tl_status tl_permute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype, void *stream,
+                           float *ms);
+
+/* Modified van Emde Boas layout.  Replaces
+ * timePermutevEB<T>(T *dev_A, uint64_t n)  (gpu/src/mveb-cycles.cuh:420). */
+tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+
+/* ---- batched queries --------------------------------------------------- */
+
+/* Replaces searchAll / timeQueryBST (gpu/src/query-bst.cuh:47,57). */
+tl_status tl_query_bst(const void *dev_A, uint64_t n, tl_dtype dtype, const void *dev_q,
+                       uint64_t nq, uint64_t *dev_ans, void *stream, float *ms);
+
+/* Replaces searchAll / timeQueryBtree (gpu/src/query-btree.cuh:64,76); any
+ * b >= 1 (the reference kernel assumes b = 32). */
+tl_status tl_query_btree(const void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                         const void *dev_q, uint64_t nq, uint64_t *dev_ans, void *stream,
+                         float *ms);
+
+/* Replaces searchAll / timeQueryvEB for a perfect mvEB
+ * (gpu/src/query-mveb.cuh:73,86). dev_table holds d entries. */
+tl_status tl_query_mveb(const void *dev_A, const tl_veb_table *dev_table, uint64_t n, uint32_t d,
+                        tl_dtype dtype, const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                        void *stream, float *ms);
+
+/* Replaces searchAll_nonperfect / timeQueryvEB_nonperfect
+ * (gpu/src/query-mveb.cuh:218,231). */
+tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                   const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                   uint64_t n, uint32_t d, uint32_t num_tables, tl_dtype dtype,
+                                   const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                                   void *stream, float *ms);
+
+/* ---- mvEB query tables -------------------------------------------------- */
+
+/* Host table build.  Replaces buildTable(vEB_table*, n, d, root_depth)
+ * (gpu/src/query-mveb.cu:25); table must hold d entries, zeroed. */
+tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth);
+
+/* Device build of the whole table set (the table-set walk of
+ * gpu/test/query-mveb-fixedN.cu:42-165 plus one buildTable per table), done
+ * by a kernel.  Outputs live in device memory:
+ *   dev_tables    : >= 64*64 entries,  dev_idx, dev_table_idx : >= 64 entries,
+ *   dev_meta[0..2]: d, num_tables, table_size.                               */
+tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
+                                 uint32_t *dev_table_idx, uint32_t *dev_meta, void *stream);
+
+/* ---- plan inspection (no device needed) -------------------------------- */
+
+/* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2): builds and bounds-
+ * checks the launch plan for (n, b, dtype) with an LDS budget of lds_bytes
+ * (0 = 64 KiB) and serializes it into buf (cap words; *used receives the
+ * words needed).  Word layout: [ops, blob_words], then 32 words per launch,
+ * then the blob of item runs and cycle tables.  Lets a test replay the plan
+ * on the host; it has no counterpart in the reference. */
+tl_status tl_plan_export(uint32_t layout, uint64_t n, uint64_t b, tl_dtype dtype,
+                         uint64_t lds_bytes, uint64_t *buf, uint64_t cap, uint64_t *used);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+"""Host replay of a serialized permutation plan (tl_plan_export).
+
+Each launch type is replayed with its *specified* effect on the array (not the
+kernel's code), using the oracle's constructions for the in-LDS sub-layouts.
+Comparing the replayed array with the oracle checks the planner's
+decomposition on the CPU; the GPU tests then check that every kernel has the
+specified effect.  Test infrastructure only."""
+import numpy as np
+
+import oracle_lib as ol
+
+OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED = range(7)
+LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV = range(4)
+MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE = range(3)
+
+
+def decode(buf):
+    nops, nblob = int(buf[0]), int(buf[1])
+    ops = [buf[2 + 32 * k: 2 + 32 * (k + 1)].astype(np.int64) for k in range(nops)]
+    blob = buf[2 + 32 * nops: 2 + 32 * nops + nblob].astype(np.int64)
+    return ops, blob
+
+
+def items(op, blob):
+    nruns, at = int(op[1]), int(op[19])
+    out = []
+    for i in range(nruns):
+        start, stride, count = (int(v) for v in blob[at + 4 * i: at + 4 * i + 3])
+        out.extend(start + j * stride for j in range(count))
+    return out
+
+
+_cache = {}
+
+
+def lds_perm(kind, n, b, rows):
+    """slot of each rank for an in-LDS item"""
+    key = (kind, n, b, rows)
+    if key not in _cache:
+        x = np.arange(n, dtype=np.int64)
+        if kind == LDS_BTREE:
+            ranks = ol.inorder_ranks(n, b).astype(np.int64)
+            pos = np.empty(n, dtype=np.int64)
+            pos[ranks] = x
+        elif kind == LDS_MVEB:
+            ranks = ol.mveb_struct_ranks(n).astype(np.int64)
+            pos = np.empty(n, dtype=np.int64)
+            pos[ranks] = x
+        elif kind == LDS_GATHER:
+            row, c = x // (b + 1), x % (b + 1)
+            pos = np.where(row >= rows, x, np.where(c == b, row, rows + row * b + c))
+        else:
+            y = x + 1
+            t = np.floor(np.log2(y)).astype(np.int64)
+            low = y - (np.int64(1) << t)
+            rev = np.array([int(format(int(v), f"0{int(tt)}b")[::-1], 2) if tt > 0 else 0
+                            for v, tt in zip(low, t)], dtype=np.int64)
+            pos = (np.int64(1) << t) + rev - 1
+        _cache[key] = pos
+    return _cache[key]
+
+
+def rev_bits(y, D):
+    out = np.zeros_like(y)
+    for i in range(D):
+        out |= ((y >> i) & 1) << (D - 1 - i)
+    return out
+
+
+def replay(buf, A):
+    ops, blob = decode(buf)
+    stash = None
+    for op in ops:
+        t = int(op[0])
+        p = [int(v) for v in op[3:11]]
+        if t == OP_LDS:
+            n, kind, b, rows = p[0], p[1], p[2], p[3]
+            pos = lds_perm(kind, n, b, rows)
+            for o in items(op, blob):
+                sub = A[o:o + n].copy()
+                A[o + pos] = sub
+        elif t == OP_REVERSE:
+            ln = p[0]
+            for o in items(op, blob):
+                A[o:o + ln] = A[o:o + ln][::-1].copy()
+        elif t == OP_BITREV:
+            base, D = np.int64(p[0]), p[1]
+            y = np.arange(1 << D, dtype=np.int64)
+            idx = base + y
+            par = base + rev_bits(y, D)
+            ok = idx >= 0
+            old = A.copy()
+            A[idx[ok]] = old[par[ok]]
+        elif t == OP_UNIT_MERGE:
+            unit, cs_at, ncyc, cu_at = p[0], p[1], p[2], p[3]
+            cs = blob[cs_at: cs_at + ncyc + 1]
+            for o in items(op, blob):
+                old = None
+                for c in range(ncyc):
+                    us = blob[cu_at + cs[c]: cu_at + cs[c + 1]]
+                    vals = [A[o + u * unit: o + (u + 1) * unit].copy() for u in us]
+                    for k in range(len(us)):
+                        dst = us[(k + 1) % len(us)]
+                        A[o + dst * unit: o + (dst + 1) * unit] = vals[k]
+        elif t == OP_SPARSE_STREAM:
+            r, l = p[0], p[1]
+            for o in items(op, blob):
+                reg = A[o:o + r * (l + 1)].reshape(r, l + 1).copy()
+                A[o:o + r] = reg[:, l]
+                A[o + r: o + r * (l + 1)] = reg[:, :l].reshape(-1)
+        elif t == OP_COPY:
+            P, stride, count, to_stash = p[0], p[1], p[2], p[3]
+            if to_stash:
+                stash = A[P: P + count * stride: stride][:count].copy()
+            else:
+                A[P: P + count * stride: stride][:count] = stash[:count]
+        elif t == OP_CHAINED:
+            kind, P, L, k, r, l = (int(v) for v in op[11:17])
+            if kind == MV_SHIFT_RIGHT:
+                A[P + k: P + k + L] = A[P:P + L].copy()
+            elif kind == MV_SHIFT_LEFT:
+                A[P - k: P - k + L] = A[P:P + L].copy()
+            else:
+                reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
+                A[P + r: P + r * (l + 1)] = reg[:, :l].reshape(-1)
+        else:
+            raise ValueError(f"unknown op {t}")
+    return A

@@ -1,0 +1,107 @@
+"""GPU parity of the batched queries against the oracle (hits and misses),
+and of the device-built mvEB table set against the oracle's."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+import tl_amd
+
+pytestmark = pytest.mark.gpu
+
+NP = {"u32": np.uint32, "u64": np.uint64, "i32": np.int32, "i64": np.int64}
+STORE = {"u32": (np.int32, torch.int32), "u64": (np.int64, torch.int64),
+         "i32": (np.int32, torch.int32), "i64": (np.int64, torch.int64)}
+SIZES = [1, 2, 3, 7, 15, 16, 17, 100, 255, 1000, 4095, 4096, 65535, 65536, 100003, 1048575,
+         1000003]
+
+
+def make_keys(kind, n, rng):
+    info = np.iinfo(NP[kind])
+    lo = max(int(info.min), -(1 << 62))
+    hi = min(int(info.max), (1 << 62))
+    keys = np.unique(rng.integers(lo, hi, size=int(n * 1.2) + 16, dtype=np.int64))
+    while keys.size < n:
+        keys = np.unique(np.concatenate([keys, rng.integers(lo, hi, size=n, dtype=np.int64)]))
+    keys = np.sort(rng.choice(keys, size=n, replace=False)).astype(NP[kind])
+    return keys
+
+
+def make_queries(kind, keys, nq, rng):
+    hits = keys[rng.integers(0, keys.size, size=nq // 2)]
+    info = np.iinfo(NP[kind])
+    lo = max(int(info.min), -(1 << 62))
+    hi = min(int(info.max), (1 << 62))
+    cand = rng.integers(lo, hi, size=nq - nq // 2, dtype=np.int64).astype(NP[kind])
+    edge = np.array([keys[0], keys[-1]], dtype=NP[kind])
+    q = np.concatenate([hits, cand, edge])
+    rng.shuffle(q)
+    return q
+
+
+def to_dev(a, kind, dev):
+    st_np, st_t = STORE[kind]
+    return torch.from_numpy(np.ascontiguousarray(a).view(st_np).copy()).to(dev)
+
+
+def gpu_answers(layout, b, kind, A_np, Q_np, dev):
+    A = to_dev(A_np, kind, dev)
+    Q = to_dev(Q_np, kind, dev)
+    ans = torch.zeros(Q_np.size, dtype=torch.int64, device=dev)
+    n = A_np.size
+    if layout == "bst":
+        tl_amd.query_bst(A, Q, ans, kind=kind)
+    elif layout == "btree":
+        tl_amd.query_btree(A, b, Q, ans, kind=kind)
+    else:
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev)
+        d, nt, ts = (int(v) for v in meta[:3].cpu())
+        if nt == 1 and n == (1 << d) - 1:
+            tl_amd.query_mveb(A, tables, d, Q, ans, kind=kind)
+        else:
+            tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, nt, Q, ans, kind=kind)
+    torch.cuda.synchronize()
+    return ans.cpu().numpy().view(np.uint64)
+
+
+QCASES = [("bst", 1), ("btree", 16), ("btree", 8), ("btree", 2), ("btree", 5), ("btree", 32),
+          ("btree", 64), ("btree", 1), ("btree", 100), ("mveb", 0)]
+
+
+@pytest.mark.parametrize("layout,b", QCASES)
+@pytest.mark.parametrize("kind", ["u32", "u64", "i32", "i64"])
+def test_query_matches_oracle(layout, b, kind, dev):
+    rng = np.random.default_rng(1234 + b)
+    for n in SIZES:
+        keys = make_keys(kind, n, rng)
+        A = ol.permute(layout, keys.copy(), b)
+        Q = make_queries(kind, keys, 4096, rng)
+        got = gpu_answers(layout, b, kind, A, Q, dev)
+        exp = ol.search(layout, A, Q, kind, b)
+        assert np.array_equal(got, exp), (layout, b, kind, n)
+        hit = got != n
+        assert np.array_equal(A[got[hit].astype(np.int64)], Q[hit])
+        assert not np.isin(Q[~hit], keys).any()
+
+
+def test_tableset_device_matches_oracle(dev):
+    for n in [1, 2, 3, 6, 7, 100, 127, 1000, 65535, 65536, 100000, 300_000_000, (1 << 29) - 1,
+              3 * 10 ** 8, (1 << 30)]:
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev)
+        d, nt, ts = (int(v) for v in meta[:3].cpu())
+        od, ont, ots, otab, oidx, otidx = ol.tableset(n)
+        assert (d, nt, ts) == (od, ont, ots), n
+        raw = tables.cpu().numpy().view(np.uint64).reshape(-1, 3)[:ts]
+        got = [(int(r[0]), int(r[1]), int(r[2]) & 0xFFFFFFFF) for r in raw]
+        assert got == otab, n
+        assert [int(v) for v in idx[:nt].cpu()] == oidx
+        assert [int(v) for v in tidx[:nt].cpu()] == otidx
+
+
+def test_config3_nonperfect_tableset_shape(dev):
+    """n = 3e8 mvEB: 4 tables of depths 29/16/7/3 (table_size 55)."""
+    tables, idx, tidx, meta = tl_amd.veb_tableset_device(3 * 10 ** 8, device=dev)
+    d, nt, ts = (int(v) for v in meta[:3].cpu())
+    assert (d, nt, ts) == (29, 4, 55)
+    n = 3 * 10 ** 8
+    assert [int(v) for v in idx[:4].cpu()] == [n - 38114, n - 119, n - 7, n]

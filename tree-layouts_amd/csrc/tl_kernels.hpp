@@ -1,0 +1,200 @@
+// tl_kernels.hpp -- gfx950 kernels of the in-place layout permutations.
+//
+// Every kernel here moves keys of width sizeof(T) (4 or 8 bytes); layouts do
+// not depend on key values, so one instantiation per width serves every key
+// type.  All global accesses are coalesced runs: a workgroup reads and writes
+// contiguous spans of at least 256 elements (the bit-reversal tiles read and
+// write rows of 2^K elements).
+#pragma once
+
+#include "tl_internal.hpp"
+
+namespace tl {
+
+// ---------------------------------------------------------------------------
+// kind of in-LDS layout: per-level bit reversal of a BST's top levels
+enum : uint32_t { LDS_LEVELREV = 3 };
+
+__device__ inline uint32_t rev_low_bits32(uint32_t x, uint32_t bits) {
+    return bits ? (__brev(x) >> (32 - bits)) : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// tiled bit-reversal involution: the key at base + y swaps with the key at
+//     base + rev_D(y), y in [0, 2^D).  y = (h | m | l) with K high bits, D-2K
+//     middle bits and K low bits; rev(y) = (rev l | rev m | rev h), so tile m
+//     (2^K rows of 2^K consecutive keys) exchanges with tile rev(m) through
+//     LDS, transposed.  A key whose index base + y is negative does not exist
+//     (the 1-based BST index 0).  Carter-Gatlin style blocking.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
+                                                      uint32_t D) {
+    constexpr int R = 1 << K;
+    __shared__ T sA[R][R + 1];
+    __shared__ T sB[R][R + 1];
+    const uint32_t MB = D - 2 * K;
+    const uint64_t m = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (MB < 64 && (m >> MB) != 0) return;
+    const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
+    if (m > mr) return;
+    const uint32_t hs = D - K;
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const uint32_t h = e >> K, l = e & (R - 1);
+        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
+        if (ia >= 0) sA[h][l] = A[ia];
+        if (m != mr) {
+            const int64_t ib = base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l);
+            sB[h][l] = A[ib];
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const uint32_t h = e >> K, l = e & (R - 1);
+        const uint32_t hr = rev_low_bits32(l, K), lr = rev_low_bits32(h, K);
+        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
+        if (m != mr) {
+            const int64_t ib = base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l);
+            A[ia] = sB[hr][lr];
+            A[ib] = sA[hr][lr];
+        } else if (ia >= 0) {
+            A[ia] = sA[hr][lr];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// decoupled chained moves.  A move copies a chunk of at most 256*PER keys
+//     from src to dst inside one array.  Moves are numbered so that every
+//     chunk whose source overlaps a move's destination has a smaller number.
+//     Workgroups take move numbers from an atomic ticket, load the chunk,
+//     publish "loaded", wait until the lower-numbered moves whose sources
+//     they overwrite have published, then store.  Every awaited move belongs
+//     to a workgroup that already started and publishes before it waits, so
+//     the scheme cannot deadlock; the wait is still bounded and reports a
+//     timeout through err instead of hanging.
+enum : uint32_t { MV_SHIFT_RIGHT = 0, MV_SHIFT_LEFT = 1, MV_SPARSE_SLIDE = 2 };
+
+struct MoveSpec {
+    uint32_t kind;
+    uint64_t P;       // region start
+    uint64_t L;       // shift: region length
+    uint64_t k;       // shift distance
+    uint64_t r, l;    // sparse slide: rows and leaf-block length
+    uint64_t CH;      // chunk length
+    uint64_t nchunk;  // chunks per region (shift) or per leaf block (slide)
+};
+
+__device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uint64_t &dst,
+                               uint64_t &len) {
+    if (m.kind == MV_SHIFT_RIGHT) {
+        const uint64_t c = m.nchunk - 1 - t;
+        src = m.P + c * m.CH;
+        len = min(m.CH, m.L - c * m.CH);
+        dst = src + m.k;
+    } else if (m.kind == MV_SHIFT_LEFT) {
+        src = m.P + t * m.CH;
+        len = min(m.CH, m.L - t * m.CH);
+        dst = src - m.k;
+    } else {
+        const uint64_t i = m.r - 1 - t / m.nchunk;
+        const uint64_t c = m.nchunk - 1 - t % m.nchunk;
+        src = m.P + i * (m.l + 1) + c * m.CH;
+        len = min(m.CH, m.l - c * m.CH);
+        dst = m.P + m.r + i * m.l + c * m.CH;
+    }
+}
+
+// bounded wait: gives up after ~2^24 polls, or at once when another
+// workgroup already gave up (then the result is reported as a timeout)
+__device__ inline bool wait_flag(uint32_t *f, uint32_t *err) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 24)) {
+            atomicOr(err, 1u);
+            return false;
+        }
+        if ((spins & 255u) == 0u &&
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            return false;
+    }
+    return true;
+}
+
+// waits for every move t' < t whose source overlaps [dst, dst+len)
+__device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
+                                 uint32_t *flags, uint32_t *err) {
+    if (m.kind == MV_SHIFT_RIGHT || m.kind == MV_SHIFT_LEFT) {
+        if (dst + len <= m.P || dst >= m.P + m.L) return;
+        const uint64_t lo = dst > m.P ? (dst - m.P) / m.CH : 0;
+        uint64_t hi = (dst + len - 1 - m.P) / m.CH;
+        if (hi >= m.nchunk) hi = m.nchunk - 1;
+        for (uint64_t c = lo; c <= hi; c++) {
+            const uint64_t tt = (m.kind == MV_SHIFT_RIGHT) ? m.nchunk - 1 - c : c;
+            if (tt < t && !wait_flag(&flags[tt], err)) return;
+        }
+        return;
+    }
+    // sparse slide: sources are the leaf blocks of rows 0..r-1
+    const uint64_t L1 = m.l + 1;
+    const uint64_t rel = dst - m.P;
+    uint64_t i_lo = rel / L1, i_hi = (rel + len - 1) / L1;
+    if (i_hi >= m.r) i_hi = m.r - 1;
+    for (uint64_t i = i_lo; i <= i_hi && i < m.r; i++) {
+        const uint64_t bs = i * L1, be = bs + m.l;   // block i source [bs, be)
+        const uint64_t ov_lo = rel > bs ? rel - bs : 0;
+        const uint64_t ov_end = (rel + len < be ? rel + len : be);
+        if (ov_end <= bs + ov_lo) continue;
+        const uint64_t c_lo = ov_lo / m.CH, c_hi = (ov_end - bs - 1) / m.CH;
+        for (uint64_t c = c_lo; c <= c_hi; c++) {
+            const uint64_t tt = (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c);
+            if (tt < t && !wait_flag(&flags[tt], err)) return;
+        }
+    }
+}
+
+template <typename T, int PER>
+__global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
+                                                 uint32_t *flags, unsigned long long *counter,
+                                                 uint32_t *err) {
+    __shared__ uint64_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        __syncthreads();
+        if (t >= nmoves) return;
+        uint64_t src, dst, len;
+        move_at(ms, t, src, dst, len);
+        T v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+            if (idx < len) v[j] = A[src + idx];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            wait_deps(ms, t, dst, len, flags, err);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+            if (idx < len) A[dst + idx] = v[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strided copy (stash / unstash of displaced keys)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_strided(T *__restrict__ dst, uint64_t dstride,
+                                                      const T *__restrict__ src, uint64_t sstride,
+                                                      uint64_t count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i * dstride] = src[i * sstride];
+}
+
+}   // namespace tl

@@ -1,0 +1,505 @@
+// tl_query.hip -- batched exact-match queries over the three layouts, and the
+// device-side mvEB table-set build, for gfx950.
+//
+// Answers are the slot of the query key or n for an absent key.  The node
+// visiting order is the reference's (cpu/src/query-bst.h:30-43,
+// cpu/src/query-btree.h:33-54, gpu/src/query-mveb.cuh:43-68,173-213), so
+// every answer equals the reference's for every query it defines and misses
+// are answered instead of reading past the array or the table.
+#include <algorithm>
+
+#include "tl_internal.hpp"
+
+namespace tl {
+
+// ---------------------------------------------------------------------------
+// BST: one lane per query.  The top TOP keys (the first levels) are staged in
+// LDS; each lane keeps two queries in flight so that two dependent descents
+// overlap their memory latency.
+constexpr uint32_t BST_TOP = 2047;
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint64_t n,
+                                                   const K *__restrict__ Q, uint64_t nq,
+                                                   uint64_t *__restrict__ ans) {
+    __shared__ K top[BST_TOP];
+    const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP, n);
+    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
+    for (uint64_t q0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; q0 < nq; q0 += stride) {
+        const bool two = q0 + 1 < nq;
+        const K qa = Q[q0];
+        const K qb = two ? Q[q0 + 1] : qa;
+        uint64_t ia = 0, ib = two ? 0 : n;
+        uint64_t ra = n, rb = n;
+        while (ia < n || ib < n) {
+            K ca, cb;
+            if (ia < n) ca = ia < ntop ? top[ia] : A[ia];
+            if (ib < n) cb = ib < ntop ? top[ib] : A[ib];
+            if (ia < n) {
+                if (qa == ca) {
+                    ra = ia;
+                    ia = n;
+                } else {
+                    ia = 2 * ia + 1 + (qa > ca);
+                }
+            }
+            if (ib < n) {
+                if (qb == cb) {
+                    rb = ib;
+                    ib = n;
+                } else {
+                    ib = 2 * ib + 1 + (qb > cb);
+                }
+            }
+        }
+        ans[q0] = ra;
+        if (two) ans[q0 + 1] = rb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B-tree, group form: G lanes (a power of two >= b, <= 64) serve one query.
+// Lane j loads key j of the node (one coalesced b-key load per level); two
+// wave ballots give "key < q" and "key == q"; the popcount of the group's
+// "<" bits is the reference's branch index j (first key that is not smaller)
+// and the "==" bit at that index is a hit.  Slots at or past n count as +inf.
+template <typename K, int G>
+__global__ __launch_bounds__(256) void k_query_btree_group(const K *__restrict__ A, uint64_t n,
+                                                           uint32_t b, const K *__restrict__ Q,
+                                                           uint64_t nq, uint64_t *__restrict__ ans) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g = lane / G, j = lane % G;
+    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1);
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint64_t waves_q = (nq + groups - 1) / groups;   // uniform trip count
+    uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    for (uint64_t it = 0; it < waves_q; it++, qi += groups) {
+        const bool valid = qi < nq;
+        K q{};
+        if (valid) q = Q[qi];
+        uint64_t i = 0, res = n;
+        bool active = valid;
+        while (__any(active)) {
+            bool lt = false, eq = false;
+            if (active && j < b && i + j < n) {
+                const K key = A[i + j];
+                lt = key < q;
+                eq = key == q;
+            }
+            const uint64_t mlt = (__ballot(lt) >> (g * G)) & gmask;
+            const uint64_t meq = (__ballot(eq) >> (g * G)) & gmask;
+            if (active) {
+                const uint32_t js = (uint32_t)__popcll(mlt);
+                if ((meq >> js) & 1ull) {
+                    res = i + js;
+                    active = false;
+                } else {
+                    i = (i + 1) * (b + 1) + (uint64_t)js * b - 1;
+                    if (i >= n) active = false;
+                }
+            }
+        }
+        if (valid && j == 0) ans[qi] = res;
+    }
+}
+
+// B-tree, lane form (b == 1 or b > 64): the reference's linear node scan
+template <typename K>
+__global__ __launch_bounds__(256) void k_query_btree_lane(const K *__restrict__ A, uint64_t n,
+                                                          uint64_t b, const K *__restrict__ Q,
+                                                          uint64_t nq, uint64_t *__restrict__ ans) {
+    for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < nq;
+         qi += (uint64_t)gridDim.x * blockDim.x) {
+        const K q = Q[qi];
+        uint64_t i = 0, res = n;
+        while (i < n) {
+            uint64_t j = 0;
+            bool hit = false;
+            for (; j < b && i + j < n; j++) {
+                const K cur = A[i + j];
+                if (q == cur) {
+                    hit = true;
+                    break;
+                }
+                if (q < cur) break;
+            }
+            if (hit) {
+                res = i + j;
+                break;
+            }
+            i = (i + 1) * (b + 1) + j * b - 1;
+        }
+        ans[qi] = res;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mvEB: one lane per query with the Brodal-Fagerberg-Jacob recurrence
+// pos[t] = pos[D[t]] + R[t] + (bfs & R[t]) * L[t]; pos[] lives in LDS as
+// [depth][lane] (32-bit when the layout has fewer than 2^32 keys) and the
+// table set is staged in LDS.  Running out of depth d_j in structure j means
+// the key is absent (the reference reads table[d] there).
+struct TblEnt {
+    uint64_t L, R;
+    uint32_t D, pad;
+};
+
+template <typename K, typename PT>
+__global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uint64_t n,
+                                                    const tl_veb_table *__restrict__ tables,
+                                                    uint32_t table_size,
+                                                    const uint64_t *__restrict__ idx,
+                                                    const uint32_t *__restrict__ tidx,
+                                                    uint32_t ntables, uint32_t dmax,
+                                                    const K *__restrict__ Q, uint64_t nq,
+                                                    uint64_t *__restrict__ ans) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TblEnt *tb = reinterpret_cast<TblEnt *>(smem_raw);
+    uint64_t *s_idx = reinterpret_cast<uint64_t *>(tb + table_size);
+    uint32_t *s_tidx = reinterpret_cast<uint32_t *>(s_idx + 64);
+    PT *pos = reinterpret_cast<PT *>(s_tidx + 64);
+    for (uint32_t i = threadIdx.x; i < table_size; i += blockDim.x) {
+        tb[i].L = tables[i].L;
+        tb[i].R = tables[i].R;
+        tb[i].D = tables[i].D;
+    }
+    for (uint32_t i = threadIdx.x; i < ntables; i += blockDim.x) {
+        s_idx[i] = idx ? idx[i] : n;
+        s_tidx[i] = tidx ? tidx[i] : 0;
+    }
+    __syncthreads();
+    const uint32_t me = threadIdx.x, W = blockDim.x;
+    for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < nq;
+         qi += (uint64_t)gridDim.x * blockDim.x) {
+        const K q = Q[qi];
+        uint64_t res = n, offset = 0;
+        for (uint32_t j = 0; j < ntables; j++) {
+            const TblEnt *t = tb + s_tidx[j];
+            const uint32_t dj = floor_log2_64(n - offset) + 1;
+            const uint64_t end = s_idx[j];
+            uint64_t bfs = 1, index = 0;
+            uint32_t cd = 0;
+            pos[me] = 1;
+            bool done = false;
+            while (index + offset < end) {
+                const K cur = A[offset + index];
+                if (q == cur) {
+                    res = offset + index;
+                    done = true;
+                    break;
+                }
+                bfs = 2 * bfs + (q > cur);
+                cd++;
+                if (cd >= dj) {
+                    done = true;
+                    break;
+                }
+                const TblEnt e = t[cd];
+                const uint64_t p = (uint64_t)pos[e.D * W + me] + e.R + (bfs & e.R) * e.L;
+                pos[cd * W + me] = (PT)p;
+                index = p - 1;
+            }
+            if (done) break;
+            offset = end;
+        }
+        ans[qi] = res;
+    }
+    (void)dmax;
+}
+
+// ---------------------------------------------------------------------------
+// device table-set build: one lane walks the chain of incomplete bottom trees
+// (gpu/test/query-mveb-fixedN.cu:42-165) and runs buildTable
+// (gpu/src/query-mveb.cu:25-47) for every structure with an explicit stack.
+__device__ void build_table_dev(tl_veb_table *table, uint64_t n, uint32_t d) {
+    struct Fr {
+        uint32_t base, d, root_depth;
+        uint64_t n;
+    };
+    Fr st[130];
+    int top = 0;
+    st[0] = {0, d, 0, n};
+    while (top >= 0) {
+        const Fr f = st[top--];
+        if (f.n <= 1) continue;
+        uint32_t rd, ld;
+        mveb_split(f.d, rd, ld);
+        const uint64_t root_n = (1ull << rd) - 1, leaf_n = (1ull << ld) - 1;
+        const uint32_t i = f.d - ld;
+        table[f.base + i].L = leaf_n;
+        table[f.base + i].R = root_n;
+        table[f.base + i].D = f.root_depth;
+        st[++top] = {f.base + i, ld, i + f.root_depth, leaf_n};
+        st[++top] = {f.base, rd, f.root_depth, root_n};
+    }
+}
+
+__global__ void k_tableset(uint64_t n, tl_veb_table *tables, uint64_t *idx, uint32_t *tidx,
+                           uint32_t *meta) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t d = floor_log2_64(n) + 1;
+    uint64_t sizes[64];
+    uint32_t depths[64];
+    uint32_t nt = 1, tsize = d;
+    sizes[0] = n;
+    depths[0] = d;
+    if (n != (1ull << d) - 1) {
+        uint64_t tn = n;
+        uint32_t td = d;
+        while (td >= 2 && nt < 64) {
+            uint32_t rd, ld;
+            mveb_split(td, rd, ld);
+            const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
+            const uint64_t nf = (tn - r) / l, inc = tn - r - nf * l;
+            if (inc == 0) break;
+            const uint32_t incd = floor_log2_64(inc) + 1;
+            sizes[nt] = inc;
+            depths[nt] = incd;
+            nt++;
+            tsize += incd;
+            tn = inc;
+            td = incd;
+            if (inc == (1ull << incd) - 1) break;
+        }
+    }
+    for (uint32_t i = 0; i < tsize; i++) tables[i] = tl_veb_table{0, 0, 0};
+    uint32_t off = 0;
+    for (uint32_t j = 0; j < nt; j++) {
+        tidx[j] = off;
+        build_table_dev(tables + off, sizes[j], depths[j]);
+        off += depths[j];
+        idx[j] = (j + 1 < nt) ? n - sizes[j + 1] : n;
+    }
+    meta[0] = d;
+    meta[1] = nt;
+    meta[2] = tsize;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static uint32_t tableset_size(uint64_t n) {
+    const uint32_t d = floor_log2_64(n) + 1;
+    uint32_t tsize = d;
+    if (n == (1ull << d) - 1) return tsize;
+    uint64_t tn = n;
+    uint32_t td = d, nt = 1;
+    while (td >= 2 && nt < 64) {
+        uint32_t rd, ld;
+        mveb_split(td, rd, ld);
+        const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
+        const uint64_t nf = (tn - r) / l, inc = tn - r - nf * l;
+        if (inc == 0) break;
+        const uint32_t incd = floor_log2_64(inc) + 1;
+        nt++;
+        tsize += incd;
+        tn = inc;
+        td = incd;
+        if (inc == (1ull << incd) - 1) break;
+    }
+    return tsize;
+}
+
+static tl_status qcheck(const void *A, uint64_t n, const void *q, uint64_t nq, const void *ans) {
+    if (nq && (!q || !ans)) {
+        tl::set_error("null query or answer buffer");
+        return TL_EINVAL;
+    }
+    if (n && !A) {
+        tl::set_error("null array");
+        return TL_EINVAL;
+    }
+    return TL_OK;
+}
+
+static uint32_t grid_for(Ctx *c, uint64_t work, uint32_t per_block) {
+    const uint64_t want = (work + per_block - 1) / per_block;
+    const uint64_t cap = (uint64_t)c->num_cus * 16;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+template <typename K>
+static void launch_bst(Ctx *c, const void *A, uint64_t n, const void *q, uint64_t nq, uint64_t *ans,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_query_bst<K>, dim3(grid_for(c, nq, 512)), dim3(256), 0, s,
+                       (const K *)A, n, (const K *)q, nq, ans);
+}
+
+template <typename K>
+static void launch_btree(Ctx *c, const void *A, uint64_t n, uint64_t b, const void *q, uint64_t nq,
+                         uint64_t *ans, hipStream_t s) {
+    if (b >= 2 && b <= 64) {
+        const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
+        const uint32_t grid = grid_for(c, nq, 256 / G);
+        switch (G) {
+#define TL_G(GG)                                                                                   \
+    case GG:                                                                                       \
+        hipLaunchKernelGGL((k_query_btree_group<K, GG>), dim3(grid), dim3(256), 0, s, (const K *)A, \
+                           n, (uint32_t)b, (const K *)q, nq, ans);                                 \
+        break;
+            TL_G(2) TL_G(4) TL_G(8) TL_G(16) TL_G(32) TL_G(64)
+#undef TL_G
+        }
+    } else {
+        hipLaunchKernelGGL(k_query_btree_lane<K>, dim3(grid_for(c, nq, 256)), dim3(256), 0, s,
+                           (const K *)A, n, b, (const K *)q, nq, ans);
+    }
+}
+
+template <typename K>
+static void launch_mveb(Ctx *c, const void *A, uint64_t n, const tl_veb_table *tables,
+                        uint32_t table_size, const uint64_t *idx, const uint32_t *tidx, uint32_t nt,
+                        const void *q, uint64_t nq, uint64_t *ans, hipStream_t s) {
+    const uint32_t dmax = floor_log2_64(n ? n : 1) + 1;
+    const bool narrow = n < (1ull << 32);
+    const size_t head = table_size * sizeof(TblEnt) + 64 * 8 + 64 * 4;
+    const size_t bytes = head + (size_t)(dmax + 1) * 256 * (narrow ? 4 : 8);
+    const uint32_t grid = grid_for(c, nq, 256);
+    if (narrow)
+        hipLaunchKernelGGL((k_query_mveb<K, uint32_t>), dim3(grid), dim3(256), bytes, s,
+                           (const K *)A, n, tables, table_size, idx, tidx, nt, dmax, (const K *)q,
+                           nq, ans);
+    else
+        hipLaunchKernelGGL((k_query_mveb<K, uint64_t>), dim3(grid), dim3(256), bytes, s,
+                           (const K *)A, n, tables, table_size, idx, tidx, nt, dmax, (const K *)q,
+                           nq, ans);
+}
+
+#define TL_DISPATCH(dt, FN, ...)                                                                 \
+    switch (dt) {                                                                                \
+    case TL_U32: FN<uint32_t>(__VA_ARGS__); break;                                               \
+    case TL_U64: FN<uint64_t>(__VA_ARGS__); break;                                               \
+    case TL_I32: FN<int32_t>(__VA_ARGS__); break;                                                \
+    case TL_I64: FN<int64_t>(__VA_ARGS__); break;                                                \
+    default: set_error("unknown dtype"); return TL_EINVAL;                                       \
+    }
+
+}   // namespace tl
+
+using namespace tl;
+
+extern "C" tl_status tl_query_bst(const void *dev_A, uint64_t n, tl_dtype dtype, const void *dev_q,
+                                  uint64_t nq, uint64_t *dev_ans, void *stream, float *ms) {
+    TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
+    if (nq == 0) return TL_OK;
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    TL_DISPATCH(dtype, launch_bst, c, dev_A, n, dev_q, nq, dev_ans, s);
+    TL_LAUNCH_CHECK();
+    return tm.stop();
+}
+
+extern "C" tl_status tl_query_btree(const void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                    const void *dev_q, uint64_t nq, uint64_t *dev_ans, void *stream,
+                                    float *ms) {
+    TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
+    if (b == 0) {
+        set_error("b must be >= 1");
+        return TL_EINVAL;
+    }
+    if (nq == 0) return TL_OK;
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    TL_DISPATCH(dtype, launch_btree, c, dev_A, n, b, dev_q, nq, dev_ans, s);
+    TL_LAUNCH_CHECK();
+    return tm.stop();
+}
+
+extern "C" tl_status tl_query_mveb(const void *dev_A, const tl_veb_table *dev_table, uint64_t n,
+                                   uint32_t d, tl_dtype dtype, const void *dev_q, uint64_t nq,
+                                   uint64_t *dev_ans, void *stream, float *ms) {
+    TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
+    if (nq == 0) return TL_OK;
+    if (n == 0 || d == 0 || d > 63 || !dev_table || n > (1ull << d) - 1) {
+        set_error("tl_query_mveb: need 1 <= n <= 2^d - 1 and a table of d entries");
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    TL_DISPATCH(dtype, launch_mveb, c, dev_A, n, dev_table, d, (const uint64_t *)nullptr,
+                (const uint32_t *)nullptr, 1u, dev_q, nq, dev_ans, s);
+    TL_LAUNCH_CHECK();
+    return tm.stop();
+}
+
+extern "C" tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                              const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                              uint64_t n, uint32_t d, uint32_t num_tables,
+                                              tl_dtype dtype, const void *dev_q, uint64_t nq,
+                                              uint64_t *dev_ans, void *stream, float *ms) {
+    TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
+    if (nq == 0) return TL_OK;
+    if (n == 0 || num_tables == 0 || num_tables > 64 || !dev_tables || !dev_idx ||
+        !dev_table_idx) {
+        set_error("tl_query_mveb_nonperfect: bad table set");
+        return TL_EINVAL;
+    }
+    // table_size is not part of the reference signature; it is fixed by n
+    // (the sum of the structure depths of the chain of incomplete trees)
+    const uint32_t table_size = tableset_size(n);
+    (void)d;
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    TL_DISPATCH(dtype, launch_mveb, c, dev_A, n, dev_tables, table_size, dev_idx, dev_table_idx,
+                num_tables, dev_q, nq, dev_ans, s);
+    TL_LAUNCH_CHECK();
+    return tm.stop();
+}
+
+extern "C" tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d,
+                                        uint32_t root_depth) {
+    if (!table || d == 0 || d > 63) {
+        set_error("tl_veb_build_table: bad arguments");
+        return TL_EINVAL;
+    }
+    // host recursion of gpu/src/query-mveb.cu:25-47
+    struct Fr {
+        tl_veb_table *t;
+        uint64_t n;
+        uint32_t d, rdp;
+    };
+    std::vector<Fr> st{{table, n, d, root_depth}};
+    while (!st.empty()) {
+        Fr f = st.back();
+        st.pop_back();
+        if (f.n <= 1) continue;
+        uint32_t rd, ld;
+        mveb_split(f.d, rd, ld);
+        const uint64_t root_n = (1ull << rd) - 1, leaf_n = (1ull << ld) - 1;
+        const uint32_t i = f.d - ld;
+        f.t[i].L = leaf_n;
+        f.t[i].R = root_n;
+        f.t[i].D = f.rdp;
+        st.push_back({f.t + i, leaf_n, ld, i + f.rdp});
+        st.push_back({f.t, root_n, rd, f.rdp});
+    }
+    return TL_OK;
+}
+
+extern "C" tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
+                                            uint32_t *dev_table_idx, uint32_t *dev_meta,
+                                            void *stream) {
+    if (n == 0 || !dev_tables || !dev_idx || !dev_table_idx || !dev_meta) {
+        set_error("tl_veb_tableset_device: bad arguments");
+        return TL_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tableset, dim3(1), dim3(64), 0, s, n, dev_tables, dev_idx, dev_table_idx,
+                       dev_meta);
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}

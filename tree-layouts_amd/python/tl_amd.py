@@ -1,0 +1,210 @@
+"""ctypes binding of libtl_amd.so (the gfx950 Tree-Layouts C ABI).
+
+Thin Python surface over include/tree_layouts.h for tests, smoke checks and
+the benchmark.  Device buffers are torch tensors on a ROCm device; keys are
+stored in torch.int32 / torch.int64 tensors and the key type is named
+explicitly ('u32', 'u64', 'i32', 'i64'), since the library compares unsigned
+or signed according to that name while layouts only depend on the width.
+
+Nothing here falls back to a CPU path: when the shared library is missing
+every call raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtl_amd.so")
+
+TL_OK, TL_EINVAL, TL_ENOMEM, TL_EHIP, TL_ETIMEOUT, TL_EUNSUPPORTED = range(6)
+DTYPE = {"u32": 0, "u64": 1, "i32": 2, "i64": 3}
+WIDTH = {"u32": 4, "u64": 8, "i32": 4, "i64": 8}
+LAYOUT = {"bst": 0, "btree": 1, "mveb": 2}
+
+
+class TLError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"tree-layouts status {status}: {msg}")
+        self.status = status
+
+
+class VebTable(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint64), ("R", ctypes.c_uint64), ("D", ctypes.c_uint32)]
+
+
+_lib = None
+
+# (name, argtypes) for every exported entry point
+_SIGS = {
+    "tl_last_error": [],
+    "tl_version": [],
+    "tl_permute_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                       ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_btree": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_mveb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                        ctypes.POINTER(ctypes.c_float)],
+    "tl_query_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                     ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                     ctypes.POINTER(ctypes.c_float)],
+    "tl_query_btree": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                       ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.POINTER(ctypes.c_float)],
+    "tl_query_mveb": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                      ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_query_mveb_nonperfect": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_float)],
+    "tl_veb_build_table": [ctypes.POINTER(VebTable), ctypes.c_uint64, ctypes.c_uint32,
+                           ctypes.c_uint32],
+    "tl_veb_tableset_device": [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "tl_plan_export": [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                       ctypes.POINTER(ctypes.c_uint64)],
+}
+_OPTIONAL = {}
+
+
+def lib():
+    """Load libtl_amd.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TLError(TL_EHIP, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_char_p if name in ("tl_last_error", "tl_version") else ctypes.c_int
+    for name, (args, res) in _OPTIONAL.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(st):
+    if st != TL_OK:
+        raise TLError(st, lib().tl_last_error().decode())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _kind(t, kind):
+    if kind is not None:
+        return kind
+    import torch
+    return {torch.int32: "i32", torch.int64: "i64"}[t.dtype]
+
+
+def _ms(timed):
+    return ctypes.byref(ctypes.c_float(0.0)) if timed else None
+
+
+def _timed_call(fn, *args, timed=True):
+    out = ctypes.c_float(0.0)
+    _check(fn(*args, ctypes.byref(out) if timed else None))
+    return out.value if timed else None
+
+
+# ---- permutations ----------------------------------------------------------
+
+def permute_bst(A, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_permute_bst, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                       timed=timed)
+
+
+def permute_btree(A, b, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_permute_btree, _ptr(A), A.numel(), b, DTYPE[k],
+                       _stream(stream), timed=timed)
+
+
+def permute_mveb(A, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_permute_mveb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                       timed=timed)
+
+
+def permute(layout, A, b=16, kind=None, stream=None, timed=True):
+    if layout == "bst":
+        return permute_bst(A, kind, stream, timed)
+    if layout == "btree":
+        return permute_btree(A, b, kind, stream, timed)
+    return permute_mveb(A, kind, stream, timed)
+
+
+# ---- queries ---------------------------------------------------------------
+
+def query_bst(A, Q, ans, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_bst, _ptr(A), A.numel(), DTYPE[k], _ptr(Q), Q.numel(),
+                       _ptr(ans), _stream(stream), timed=timed)
+
+
+def query_btree(A, b, Q, ans, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_btree, _ptr(A), A.numel(), b, DTYPE[k], _ptr(Q),
+                       Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+
+
+def query_mveb(A, table, d, Q, ans, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_mveb, _ptr(A), _ptr(table), A.numel(), d, DTYPE[k],
+                       _ptr(Q), Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+
+
+def query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None, stream=None,
+                          timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_mveb_nonperfect, _ptr(A), _ptr(tables), _ptr(idx),
+                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _ptr(Q), Q.numel(),
+                       _ptr(ans), _stream(stream), timed=timed)
+
+
+def veb_tableset_device(n, device="cuda", stream=None):
+    """Device-built mvEB table set: (tables int64[64*64*3], idx, tidx, meta) tensors."""
+    import torch
+    tables = torch.zeros(64 * 64 * 3, dtype=torch.int64, device=device)   # 24-byte entries
+    idx = torch.zeros(64, dtype=torch.int64, device=device)
+    tidx = torch.zeros(64, dtype=torch.int32, device=device)
+    meta = torch.zeros(4, dtype=torch.int32, device=device)
+    _check(lib().tl_veb_tableset_device(n, _ptr(tables), _ptr(idx), _ptr(tidx), _ptr(meta),
+                                        _stream(stream)))
+    return tables, idx, tidx, meta
+
+
+def veb_build_table(n, d, root_depth=0):
+    arr = (VebTable * max(d, 1))()
+    _check(lib().tl_veb_build_table(arr, n, d, root_depth))
+    return [(e.L, e.R, e.D) for e in arr]
+
+
+def plan_export(layout, n, b=1, kind="u64", lds_bytes=0):
+    """Serialized dry-run plan of a permutation (list of uint64 words)."""
+    import numpy as np
+    used = ctypes.c_uint64(0)
+    L = lib()
+    _check(L.tl_plan_export(LAYOUT[layout], n, b, DTYPE[kind], lds_bytes, None, 0,
+                            ctypes.byref(used)))
+    buf = np.zeros(used.value, dtype=np.uint64)
+    _check(L.tl_plan_export(LAYOUT[layout], n, b, DTYPE[kind], lds_bytes,
+                            buf.ctypes.data_as(ctypes.c_void_p), used.value,
+                            ctypes.byref(used)))
+    return buf
