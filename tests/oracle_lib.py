@@ -2,6 +2,7 @@
 built in this container, to the compiled reference CPU code
 (oracle/_ref/libref.so).  Test infrastructure: used only as the checker."""
 import ctypes
+ctypes = ctypes
 import os
 
 import numpy as np
