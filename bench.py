@@ -1,0 +1,205 @@
+"""Benchmark of the Tree-Layouts hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): level-order
+B-tree, b = 16, n = 2^30 uint64 keys from the cumulative sum of
+max(1, ceil(lognormal(0, 2))) gaps (torch CUDA generator, seed 4), in-place
+permute, then 2^27 uniform-hit queries answered by plain root descent and by
+the learned start-node descent.
+
+A step is one in-place permute of the whole array.  The permutation only
+depends on (n, b), so repeating it on the same buffer repeats exactly the
+same data movement; the first permute (from sorted input) is the one whose
+result is checked.  Queries are timed separately on a freshly built layout.
+
+Prints one JSON line (see the keys below); --gpus N > 1 is rejected because
+this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tree-layouts_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen_keys(n, seed, dev):
+    import torch
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    chunk = 1 << 27
+    carry = 0
+    for s in range(0, n, chunk):
+        m = min(chunk, n - s)
+        gaps = torch.empty(m, dtype=torch.float64, device=dev).log_normal_(0.0, 2.0, generator=g)
+        gaps = torch.clamp(torch.ceil(gaps), min=1.0, max=float(1 << 40)).to(torch.int64)
+        part = torch.cumsum(gaps, 0) + carry
+        keys[s:s + m] = part
+        carry = int(part[-1].item())
+        del gaps, part
+    return keys, g
+
+
+def sample_check_layout(A, keys_sorted, n, b, rng, samples=200_000):
+    """slot of sampled ranks from the oracle's closed form; checker only"""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    o = ol.oracle()
+    ranks = rng.integers(0, n, size=samples, dtype=np.int64)
+    pos = np.array([o.or_btree_pos(int(x), n, b) for x in ranks], dtype=np.int64)
+    import torch
+    got = A[torch.from_numpy(pos).to(A.device)].cpu().numpy()
+    exp = keys_sorted[torch.from_numpy(ranks).to(A.device)].cpu().numpy()
+    return bool(np.array_equal(got, exp))
+
+
+def cpu_baseline(keys_np, b, nq_sample, rng):
+    """Oracle OpenMP restatement of the reference CPU path (btree-cycles +
+    query-btree), on a bounded sample, on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    cores = len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, 16))
+    n = keys_np.size
+    times = []
+    for _ in range(3):
+        a = keys_np.copy()
+        t0 = time.perf_counter()
+        ol.permute("btree", a, b, p=cores)
+        times.append(time.perf_counter() - t0)
+    q = keys_np[rng.integers(0, n, size=nq_sample)]
+    t0 = time.perf_counter()
+    ans = ol.search("btree", a, q, "u64", b, p=cores)
+    tq = time.perf_counter() - t0
+    assert (ans < n).all()
+    return {"permute_gkeys_s": n / min(times) / 1e9, "query_mq_s": nq_sample / tq / 1e6,
+            "cores": cores, "n": n, "nq": nq_sample}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=30)
+    ap.add_argument("--log2q", type=int, default=27)
+    ap.add_argument("--b", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-learned", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the layouts would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import tl_amd
+    dev = torch.device("cuda:0")
+    n, nq, b = 1 << args.log2n, 1 << args.log2q, args.b
+    rng = np.random.default_rng(4)
+    log(f"[bench] generating n={n} keys and {nq} queries")
+    keys, g = gen_keys(n, 4, dev)
+    qidx = torch.randint(0, n, (nq,), generator=g, device=dev, dtype=torch.int64)
+    Q = keys[qidx]
+    del qidx
+    A = keys.clone()
+    stream = torch.cuda.current_stream()
+
+    # ---- correctness of the first permute (from sorted input)
+    log("[bench] first permute + layout check")
+    tl_amd.permute_btree(A, b, kind="u64")
+    torch.cuda.synchronize()
+    layout_ok = sample_check_layout(A, keys, n, b, rng)
+    log(f"[bench] layout sample check: {layout_ok}")
+
+    # ---- timed permutes (same buffer, same data movement every step)
+    for _ in range(args.warmup):
+        tl_amd.permute_btree(A, b, kind="u64", timed=False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        tl_amd.permute_btree(A, b, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t_wall) * 1e3
+    perm_ms = ev0.elapsed_time(ev1) / args.steps
+    log(f"[bench] permute {perm_ms:.3f} ms/step (host wall {wall_ms / args.steps:.3f})")
+
+    # ---- rebuild the true layout for the queries
+    A.copy_(keys)
+    tl_amd.permute_btree(A, b, kind="u64")
+    ans = torch.empty(nq, dtype=torch.int64, device=dev)
+    for _ in range(max(1, args.warmup)):
+        tl_amd.query_btree(A, b, Q, ans, kind="u64", timed=False)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        tl_amd.query_btree(A, b, Q, ans, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    q_ms = ev0.elapsed_time(ev1) / args.steps
+    plain_ok = bool(((ans >= 0) & (ans < n)).all().item()) and bool(
+        torch.equal(A[ans.clamp(0, n - 1)], Q))
+    log(f"[bench] plain query {q_ms:.3f} ms ({nq / q_ms / 1e3:.1f} Mq/s) ok={plain_ok}")
+
+    learned = None
+    if not args.skip_learned and hasattr(tl_amd, "learned_fit"):
+        learned = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample")
+        ns = 1 << 25
+        keys_np = keys[:ns].cpu().numpy()
+        cpu = cpu_baseline(keys_np, b, 1 << 22, rng)
+        log(f"[bench] CPU {cpu}")
+
+    algo_bytes = 2.0 * n * 8
+    achieved = algo_bytes / (perm_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
+        "value": round(n / (perm_ms * 1e-3) / 1e9, 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(perm_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: lognormal(0,2) gaps, torch CUDA generator seed 4",
+        "config": {"workload": "level-order B-tree b=16, in-place permute of n=2^30 uint64 keys; "
+                               "2^27 uniform-hit queries (plain and learned start)",
+                   "n": n, "b": b, "queries": nq},
+        "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
+                    "learned": learned},
+        "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "whole tl_permute_btree call (all its launches)"},
+        "cpu_baseline": None if cpu is None else {
+            "value": round(cpu["permute_gkeys_s"], 4), "unit": "Gkeys/s", "cores": cpu["cores"],
+            "kind": "port",
+            "sample": f"oracle OpenMP restatement of btree-cycles, n=2^25 prefix of the same "
+                      f"keys, best of 3; query {cpu['query_mq_s']:.2f} Mq/s on 2^22 hits"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

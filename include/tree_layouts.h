@@ -99,6 +99,28 @@ tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d, uint32
 tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
                                  uint32_t *dev_table_idx, uint32_t *dev_meta, void *stream);
 
+/* ---- learned start node (no counterpart in the reference) -------------- */
+
+/* Bytes of a model with num_leaves leaf models (64-byte header + 32 per leaf). */
+uint64_t tl_learned_model_bytes(uint32_t num_leaves);
+
+/* Fits the two-layer rank model (root linear map -> num_leaves linear leaf
+ * maps, FP32 inference, per-leaf max rank error) on the GPU from the sorted
+ * keys dev_sorted[0..n) into dev_model (tl_learned_model_bytes bytes of
+ * device memory).  The largest leaf error is the uint32 at byte offset 32 of
+ * the model.  Addition on top of the reference; its queries answer exactly
+ * like tl_query_btree. */
+tl_status tl_learned_fit(const void *dev_sorted, uint64_t n, tl_dtype dtype, uint32_t num_leaves,
+                         void *dev_model, void *stream, float *ms);
+
+/* tl_query_btree with the descent started at the deepest node whose subtree
+ * holds the model's rank window; queries the window does not settle are
+ * answered by the plain root descent.  dev_stats (optional, one uint64,
+ * accumulated) counts those fallbacks.  2 <= b <= 64. */
+tl_status tl_query_btree_learned(const void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                 const void *dev_model, const void *dev_q, uint64_t nq,
+                                 uint64_t *dev_ans, uint64_t *dev_stats, void *stream, float *ms);
+
 /* ---- plan inspection (no device needed) -------------------------------- */
 
 /* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2): builds and bounds-

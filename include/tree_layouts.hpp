@@ -1,0 +1,156 @@
+// tree_layouts.hpp -- the reference's C++ surface over the gfx950 C ABI.
+//
+// Same names, argument order and meaning as the reference's GPU headers
+// (README.md:31-67; gpu/src/bst-hybrid.cuh:120, gpu/src/btree-cycles.cuh:77,
+// gpu/src/mveb-cycles.cuh:420, gpu/src/query-bst.cuh:57,
+// gpu/src/query-btree.cuh:76, gpu/src/query-mveb.cuh:86,231,
+// gpu/src/query-mveb.cu:25), so a reference driver switches by changing its
+// includes.  Differences, all deliberate:
+//   * one header for every layout (the reference needs one translation unit
+//     per layout header because their helper templates collide);
+//   * the timeQuery* functions draw their hit queries from a seeded
+//     counter-based generator (an extra `seed` argument, default 0) instead of
+//     rand() seeded with time(NULL), and they also compute a miss answer (n)
+//     for absent keys;
+//   * failures are reported on stderr and the functions return -1 ms (the
+//     reference has no error path at all).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "tree_layouts.h"
+
+typedef tl_veb_table vEB_table;
+
+namespace tl_detail {
+template <typename T> struct dtype_of;
+template <> struct dtype_of<uint32_t> { static constexpr tl_dtype v = TL_U32; };
+template <> struct dtype_of<uint64_t> { static constexpr tl_dtype v = TL_U64; };
+template <> struct dtype_of<int32_t> { static constexpr tl_dtype v = TL_I32; };
+template <> struct dtype_of<int64_t> { static constexpr tl_dtype v = TL_I64; };
+
+inline bool ok(tl_status s, const char *what) {
+    if (s == TL_OK) return true;
+    fprintf(stderr, "%s failed (%d): %s\n", what, (int)s, tl_last_error());
+    return false;
+}
+
+inline uint64_t mix(uint64_t seed, uint64_t i) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (i + 1) * 0xD1B54A32D192ED03ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// the reference's createRandomQueries (gpu/src/common.cuh:44-60): hits drawn
+// from the host copy of the layout
+template <typename T>
+std::vector<T> random_hits(const T *A, uint64_t n, uint64_t q, uint64_t seed) {
+    std::vector<T> out(q);
+    for (uint64_t i = 0; i < q; i++) out[i] = A[mix(seed, i) % n];
+    return out;
+}
+
+// uploads A and the queries, runs `search`, returns its kernel time
+template <typename T, typename F>
+float run_queries(T *A, T *dev_A, uint64_t n, uint64_t q, uint64_t seed, F search) {
+    std::vector<T> hq = random_hits(A, n, q, seed);
+    T *dq = nullptr;
+    uint64_t *dans = nullptr;
+    if (hipMalloc(&dq, q * sizeof(T)) != hipSuccess) return -1.f;
+    if (hipMalloc(&dans, q * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipFree(dq);
+        return -1.f;
+    }
+    float ms = -1.f;
+    if (hipMemcpy(dev_A, A, n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(dq, hq.data(), q * sizeof(T), hipMemcpyHostToDevice) == hipSuccess) {
+        float t = 0.f;
+        if (search(dq, dans, &t)) ms = t;
+    }
+    (void)hipFree(dq);
+    (void)hipFree(dans);
+    return ms;
+}
+}   // namespace tl_detail
+
+// ---- permutations (in place on dev_A, synchronous, return milliseconds) ----
+
+template <typename T>
+float timePermuteBST(T *dev_A, uint64_t n) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_bst(dev_A, n, tl_detail::dtype_of<T>::v, nullptr, &ms),
+                       "timePermuteBST"))
+        return -1.f;
+    return ms;
+}
+
+template <typename T>
+double timePermuteBtree(T *dev_A, uint64_t n, uint64_t b) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_btree(dev_A, n, b, tl_detail::dtype_of<T>::v, nullptr, &ms),
+                       "timePermuteBtree"))
+        return -1.0;
+    return ms;
+}
+
+template <typename T>
+double timePermutevEB(T *dev_A, uint64_t n) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_mveb(dev_A, n, tl_detail::dtype_of<T>::v, nullptr, &ms),
+                       "timePermutevEB"))
+        return -1.0;
+    return ms;
+}
+
+// ---- queries (hits drawn from the host layout A; kernel time in ms) ---------
+
+template <typename T>
+float timeQueryBST(T *A, T *dev_A, uint64_t n, uint64_t q, uint64_t seed = 0) {
+    return tl_detail::run_queries(A, dev_A, n, q, seed, [&](T *dq, uint64_t *dans, float *t) {
+        return tl_detail::ok(
+            tl_query_bst(dev_A, n, tl_detail::dtype_of<T>::v, dq, q, dans, nullptr, t),
+            "timeQueryBST");
+    });
+}
+
+template <typename T>
+float timeQueryBtree(T *A, T *dev_A, uint64_t n, uint64_t b, uint64_t q, uint64_t seed = 0) {
+    return tl_detail::run_queries(A, dev_A, n, q, seed, [&](T *dq, uint64_t *dans, float *t) {
+        return tl_detail::ok(
+            tl_query_btree(dev_A, n, b, tl_detail::dtype_of<T>::v, dq, q, dans, nullptr, t),
+            "timeQueryBtree");
+    });
+}
+
+template <typename T>
+float timeQueryvEB(T *A, T *dev_A, vEB_table *dev_table, uint64_t n, uint32_t d, uint64_t q,
+                   uint64_t seed = 0) {
+    return tl_detail::run_queries(A, dev_A, n, q, seed, [&](T *dq, uint64_t *dans, float *t) {
+        return tl_detail::ok(tl_query_mveb(dev_A, dev_table, n, d, tl_detail::dtype_of<T>::v, dq,
+                                           q, dans, nullptr, t),
+                             "timeQueryvEB");
+    });
+}
+
+template <typename T>
+float timeQueryvEB_nonperfect(T *A, T *dev_A, vEB_table *dev_tables, uint64_t *dev_idx,
+                              uint32_t *dev_table_idx, uint64_t n, uint32_t d,
+                              uint32_t num_tables, uint64_t q, uint64_t seed = 0) {
+    return tl_detail::run_queries(A, dev_A, n, q, seed, [&](T *dq, uint64_t *dans, float *t) {
+        return tl_detail::ok(
+            tl_query_mveb_nonperfect(dev_A, dev_tables, dev_idx, dev_table_idx, n, d, num_tables,
+                                     tl_detail::dtype_of<T>::v, dq, q, dans, nullptr, t),
+            "timeQueryvEB_nonperfect");
+    });
+}
+
+// ---- mvEB table (gpu/src/query-mveb.cu:25) ----------------------------------
+
+inline void buildTable(vEB_table *table, uint64_t n, uint32_t d, uint32_t root_depth) {
+    tl_detail::ok(tl_veb_build_table(table, n, d, root_depth), "buildTable");
+}

@@ -1,1 +1,460 @@
-// placeholder
+// tl_learned.hip -- learned start node for level-order B-tree queries (gfx950).
+//
+// Model: a recursive model index of two layers on the (key, rank) pairs of the
+// sorted array.  The root is one linear map key -> leaf; each of the
+// num_leaves leaves is a linear map key -> rank with an integer base (first
+// key and first rank of the leaf) and an FP32 slope / intercept on the key
+// offset, plus the largest rank error seen on its own keys.  Fitting is done
+// on the GPU: FP64 least-squares sums by block reductions (root), one
+// workgroup per leaf for the leaf sums and the error pass.  Inference is FP32.
+//
+// Query: leaf -> predicted rank window [pred - err, pred + err] (clipped to
+// the leaf's ranks) -> slots of the window ends through the layout's
+// rank -> slot closed form -> lowest common ancestor node -> exact group
+// descent from that node.  A key inside the window is in that subtree, so
+// hits are found there; anything else (a miss, an empty leaf) falls back to
+// the plain root descent, which decides the answer.  Answers therefore equal
+// tl_query_btree's bit for bit.
+#include <algorithm>
+
+#include "tl_internal.hpp"
+
+namespace tl {
+
+struct LearnedHeader {   // 64 bytes at the start of the model buffer
+    uint64_t kmin;       // order-preserving unsigned image of the smallest key
+    uint64_t n;
+    float ra, rc;        // leaf = floor(ra * x + rc), x = (key - kmin) * inv
+    float inv;           // 1 / (kmax - kmin)
+    uint32_t num_leaves;
+    uint32_t max_err;
+    uint32_t dtype;
+    uint64_t fallbacks;  // unused by the fit, kept for symmetry
+    uint64_t pad[2];
+};
+
+struct LeafModel {       // 32 bytes
+    uint64_t base_key;   // unsigned image of the first key of the leaf
+    uint64_t base_rank;  // its rank
+    float slope, icept;  // rank - base_rank ~ slope * (key - base_key) + icept
+    uint32_t err;        // max |prediction - rank| over the leaf's keys
+    uint32_t count;      // keys in the leaf
+};
+
+static_assert(sizeof(LearnedHeader) == 64, "header size");
+static_assert(sizeof(LeafModel) == 32, "leaf size");
+
+// order-preserving map of every key type to uint64
+template <typename K> __device__ __host__ inline uint64_t ukey(K k);
+template <> __device__ __host__ inline uint64_t ukey<uint32_t>(uint32_t k) { return k; }
+template <> __device__ __host__ inline uint64_t ukey<uint64_t>(uint64_t k) { return k; }
+template <> __device__ __host__ inline uint64_t ukey<int32_t>(int32_t k) {
+    return (uint64_t)((int64_t)k) ^ (1ull << 63);
+}
+template <> __device__ __host__ inline uint64_t ukey<int64_t>(int64_t k) {
+    return (uint64_t)k ^ (1ull << 63);
+}
+
+__device__ inline uint32_t leaf_of(uint64_t u, const LearnedHeader &h) {
+    const float x = (float)(u - h.kmin) * h.inv;
+    const float f = __fmaf_rn(h.ra, x, h.rc);
+    int l = (int)floorf(f);
+    l = l < 0 ? 0 : l;
+    return (uint32_t)min((uint32_t)l, h.num_leaves - 1);
+}
+
+__device__ inline int64_t predict(uint64_t u, const LeafModel &m) {
+    const float dx = (float)(int64_t)(u - m.base_key);
+    const float p = __fmaf_rn(m.slope, dx, m.icept);
+    return (int64_t)m.base_rank + (int64_t)rintf(p);
+}
+
+// ---------------------------------------------------------------------------
+// fit
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_root_sums(const K *__restrict__ keys, uint64_t n,
+                                                   uint64_t kmin, double inv, double *part) {
+    double s[5] = {0, 0, 0, 0, 0};   // x, y, xx, xy, count
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const double x = (double)(ukey<K>(keys[i]) - kmin) * inv;
+        const double y = (double)i / (double)n;
+        s[0] += x;
+        s[1] += y;
+        s[2] += x * x;
+        s[3] += x * y;
+        s[4] += 1.0;
+    }
+    __shared__ double red[5][256];
+    for (int k = 0; k < 5; k++) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; k++) part[(uint64_t)blockIdx.x * 5 + k] = red[k][0];
+}
+
+__global__ void k_root_solve(const double *part, uint32_t nparts, uint64_t kmin_u, double range,
+                             uint64_t n, uint32_t num_leaves, uint32_t dtype, LearnedHeader *h) {
+    if (threadIdx.x != 0) return;
+    double s[5] = {0, 0, 0, 0, 0};
+    for (uint32_t p = 0; p < nparts; p++)
+        for (int k = 0; k < 5; k++) s[k] += part[p * 5 + k];
+    const double c = s[4];
+    const double mx = s[0] / c, my = s[1] / c;
+    const double var = s[2] / c - mx * mx, cov = s[3] / c - mx * my;
+    const double a = var > 0 ? cov / var : 1.0;
+    const double b0 = my - a * mx;
+    LearnedHeader hh{};
+    hh.kmin = kmin_u;
+    hh.n = n;
+    hh.ra = (float)(a * num_leaves);
+    hh.rc = (float)(b0 * num_leaves);
+    hh.inv = range > 0 ? (float)(1.0 / range) : 0.f;
+    hh.num_leaves = num_leaves;
+    hh.max_err = 0;
+    hh.dtype = dtype;
+    *h = hh;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_leaf_bounds(const K *__restrict__ keys, uint64_t n,
+                                                     const LearnedHeader *__restrict__ hp,
+                                                     uint64_t *first) {
+    const LearnedHeader h = *hp;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t li = leaf_of(ukey<K>(keys[i]), h);
+        const int64_t lp = i ? (int64_t)leaf_of(ukey<K>(keys[i - 1]), h) : -1;
+        for (int64_t l = lp + 1; l <= (int64_t)li; l++) first[l] = i;
+        if (i == n - 1)
+            for (uint32_t l = li + 1; l <= h.num_leaves; l++) first[l] = n;
+    }
+}
+
+// one workgroup per leaf: least squares of (rank - base_rank) on the FP32 key
+// offset, then the largest error of the FP32 prediction on the leaf's keys
+template <typename K>
+__global__ __launch_bounds__(256) void k_leaf_fit(const K *__restrict__ keys,
+                                                  const uint64_t *__restrict__ first,
+                                                  LearnedHeader *h, LeafModel *leaves) {
+    const uint32_t l = blockIdx.x;
+    const uint64_t lo = first[l], hi = first[l + 1];
+    __shared__ double red[4][256];
+    __shared__ LeafModel sm;
+    const uint64_t base = lo < hi ? ukey<K>(keys[lo]) : 0;
+    double s[4] = {0, 0, 0, 0};
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const double dx = (double)(float)(int64_t)(ukey<K>(keys[i]) - base);
+        const double dy = (double)(i - lo);
+        s[0] += dx;
+        s[1] += dy;
+        s[2] += dx * dx;
+        s[3] += dx * dy;
+    }
+    for (int k = 0; k < 4; k++) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double c = (double)(hi - lo);
+        LeafModel m{};
+        m.base_key = base;
+        m.base_rank = lo;
+        m.count = (uint32_t)(hi - lo);
+        if (c > 0) {
+            const double mx = red[0][0] / c, my = red[1][0] / c;
+            const double var = red[2][0] / c - mx * mx, cov = red[3][0] / c - mx * my;
+            const double a = var > 1e-30 ? cov / var : 0.0;
+            m.slope = (float)a;
+            m.icept = (float)(my - a * mx);
+        }
+        sm = m;
+    }
+    __syncthreads();
+    const LeafModel m = sm;
+    uint32_t e = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const int64_t d = predict(ukey<K>(keys[i]), m) - (int64_t)i;
+        const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
+        e = max(e, (uint32_t)min(ad, (uint64_t)0xFFFFFFFFu));
+    }
+    __shared__ uint32_t ered[256];
+    ered[threadIdx.x] = e;
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) ered[threadIdx.x] = max(ered[threadIdx.x], ered[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        LeafModel out = m;
+        out.err = ered[0];
+        leaves[l] = out;
+        atomicMax(&h->max_err, ered[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// query
+
+struct BtreeConst {   // rank -> slot closed form of the searched layout
+    uint64_t n, b, k1;
+    uint64_t perfect, nint, span, rem, nsep;
+};
+
+__device__ inline uint64_t slot_of_rank(uint64_t x, const BtreeConst &c) {
+    uint64_t n = c.n;
+    if (!c.perfect) {
+        if (x < c.span) {
+            const uint64_t row = x / c.k1, cc = x - row * c.k1;
+            if (cc < c.b) return c.nint + row * c.b + cc;
+            x = row;
+        } else if (x < c.span + c.rem) {
+            return c.nint + (c.span - c.nsep) + (x - c.span);
+        } else {
+            x = c.nsep + (x - c.span - c.rem);
+        }
+        n = c.nint;
+    }
+    uint64_t r = x + 1, div = 1, q = r / c.k1;
+    while (r == q * c.k1) {
+        r = q;
+        div *= c.k1;
+        q = r / c.k1;
+    }
+    const uint64_t a = r - q * c.k1;
+    return (n + 1) / (div * c.k1) - 1 + q * c.b + a - 1;
+}
+
+template <typename K, int G>
+__device__ inline uint64_t group_descent(const K *__restrict__ A, uint64_t n, uint32_t b, K q,
+                                         uint64_t i, bool active, uint32_t g, uint32_t j,
+                                         bool &found) {
+    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1);
+    uint64_t res = n;
+    found = false;
+    while (__any(active)) {
+        bool lt = false, eq = false;
+        if (active && j < b && i + j < n) {
+            const K key = A[i + j];
+            lt = key < q;
+            eq = key == q;
+        }
+        const uint64_t mlt = (__ballot(lt) >> (g * G)) & gmask;
+        const uint64_t meq = (__ballot(eq) >> (g * G)) & gmask;
+        if (active) {
+            const uint32_t js = (uint32_t)__popcll(mlt);
+            if ((meq >> js) & 1ull) {
+                res = i + js;
+                found = true;
+                active = false;
+            } else {
+                i = (i + 1) * (b + 1) + (uint64_t)js * b - 1;
+                if (i >= n) active = false;
+            }
+        }
+    }
+    return res;
+}
+
+template <typename K, int G>
+__global__ __launch_bounds__(256) void k_query_btree_learned(
+    const K *__restrict__ A, BtreeConst bc, const LearnedHeader *__restrict__ hp,
+    const LeafModel *__restrict__ leaves, const K *__restrict__ Q, uint64_t nq,
+    uint64_t *__restrict__ ans, unsigned long long *stats) {
+    const LearnedHeader h = *hp;
+    const uint64_t n = bc.n;
+    const uint32_t b = (uint32_t)bc.b;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g = lane / G, j = lane % G;
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint64_t trips = (nq + groups - 1) / groups;
+    uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    uint32_t fb = 0;
+    for (uint64_t it = 0; it < trips; it++, qi += groups) {
+        const bool valid = qi < nq;
+        K q{};
+        if (valid) q = Q[qi];
+        // start node from the model (uniform inside the group)
+        uint64_t start = 0;
+        bool use_model = false;
+        if (valid) {
+            const uint64_t u = ukey<K>(q);
+            const LeafModel m = leaves[leaf_of(u, h)];
+            if (m.count) {
+                const int64_t pred = predict(u, m);
+                int64_t lo = pred - (int64_t)m.err, hi = pred + (int64_t)m.err;
+                lo = max(lo, (int64_t)m.base_rank);
+                hi = min(hi, (int64_t)(m.base_rank + m.count - 1));
+                if (lo <= hi) {
+                    uint64_t ua = slot_of_rank((uint64_t)lo, bc) / b;
+                    uint64_t ub = slot_of_rank((uint64_t)hi, bc) / b;
+                    while (ua != ub) {   // deeper node has the larger heap index
+                        if (ua > ub)
+                            ua = (ua - 1) / (b + 1);
+                        else
+                            ub = (ub - 1) / (b + 1);
+                    }
+                    start = ua * b;
+                    use_model = true;
+                }
+            }
+        }
+        bool found;
+        uint64_t res = group_descent<K, G>(A, n, b, q, start, valid, g, j, found);
+        // a descent that started below the root and did not find the key is
+        // redone from the root: the plain descent decides every other case
+        const bool again = valid && start != 0 && !found;
+        if (valid && j == 0 && (again || !use_model)) fb++;
+        const uint64_t res2 = group_descent<K, G>(A, n, b, q, 0, again, g, j, found);
+        if (again) res = res2;
+        if (valid && j == 0) ans[qi] = res;
+    }
+    if (stats) {
+        for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
+        if (lane == 0 && fb) atomicAdd(&stats[0], (unsigned long long)fb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+template <typename K>
+static tl_status fit(Ctx *c, const void *dev_sorted, uint64_t n, uint32_t num_leaves,
+                     uint32_t dtype, void *dev_model, hipStream_t s) {
+    const K *keys = (const K *)dev_sorted;
+    K kmin_k, kmax_k;
+    TL_HIP(hipMemcpyAsync(&kmin_k, keys, sizeof(K), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(&kmax_k, keys + (n - 1), sizeof(K), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    const uint64_t kmin_u = ukey<K>(kmin_k), kmax_u = ukey<K>(kmax_k);
+    if (kmax_u < kmin_u) {
+        set_error("tl_learned_fit: keys are not sorted");
+        return TL_EINVAL;
+    }
+    const double range = (double)(kmax_u - kmin_u);
+    const double inv = range > 0 ? 1.0 / range : 0.0;
+    const uint32_t nparts = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)c->num_cus * 8);
+    double *part;
+    TL_TRY(scratch(c, 3, (size_t)nparts * 5 * sizeof(double) + (num_leaves + 1) * 8ull,
+                   (void **)&part));
+    uint64_t *first = (uint64_t *)(part + (size_t)nparts * 5);
+    LearnedHeader *h = (LearnedHeader *)dev_model;
+    LeafModel *leaves = (LeafModel *)((char *)dev_model + sizeof(LearnedHeader));
+    hipLaunchKernelGGL(k_root_sums<K>, dim3(nparts), dim3(256), 0, s, keys, n, kmin_u, inv, part);
+    TL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_root_solve, dim3(1), dim3(64), 0, s, part, nparts, kmin_u, range, n,
+                       num_leaves, dtype, h);
+    TL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_leaf_bounds<K>, dim3(nparts), dim3(256), 0, s, keys, n,
+                       (const LearnedHeader *)h, first);
+    TL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_leaf_fit<K>, dim3(num_leaves), dim3(256), 0, s, keys,
+                       (const uint64_t *)first, h, leaves);
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
+template <typename K>
+static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void *model,
+                       const void *q, uint64_t nq, uint64_t *ans, unsigned long long *stats,
+                       hipStream_t s) {
+    BtreeConst bc{};
+    bc.n = n;
+    bc.b = b;
+    bc.k1 = b + 1;
+    const uint32_t h = ilog_base(n, b + 1);
+    const uint64_t P = ipow64(b + 1, h);
+    bc.perfect = (n == P * (b + 1) - 1);
+    if (bc.perfect) {
+        bc.nint = n;
+    } else {
+        const uint64_t nint = P - 1, nleaf = n - nint;
+        const uint64_t rem = nleaf % b, l = nleaf - rem, i = l / b;
+        bc.nint = nint;
+        bc.span = l + i;
+        bc.rem = rem;
+        bc.nsep = i;
+    }
+    const LearnedHeader *hp = (const LearnedHeader *)model;
+    const LeafModel *leaves = (const LeafModel *)((const char *)model + sizeof(LearnedHeader));
+    const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
+    const uint64_t want = (nq + 256 / G - 1) / (256 / G);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));
+    switch (G) {
+#define TL_G(GG)                                                                                    \
+    case GG:                                                                                        \
+        hipLaunchKernelGGL((k_query_btree_learned<K, GG>), dim3(grid), dim3(256), 0, s,             \
+                           (const K *)A, bc, hp, leaves, (const K *)q, nq, ans, stats);             \
+        break;
+        TL_G(2) TL_G(4) TL_G(8) TL_G(16) TL_G(32) TL_G(64)
+#undef TL_G
+    }
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
+}   // namespace tl
+
+using namespace tl;
+
+extern "C" uint64_t tl_learned_model_bytes(uint32_t num_leaves) {
+    return sizeof(LearnedHeader) + (uint64_t)num_leaves * sizeof(LeafModel);
+}
+
+extern "C" tl_status tl_learned_fit(const void *dev_sorted, uint64_t n, tl_dtype dtype,
+                                    uint32_t num_leaves, void *dev_model, void *stream,
+                                    float *ms) {
+    if (!dev_sorted || !dev_model || n < 2 || num_leaves == 0 || num_leaves > (1u << 24)) {
+        set_error("tl_learned_fit: need n >= 2 sorted keys, 1..2^24 leaves and a model buffer");
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    tl_status st;
+    switch (dtype) {
+    case TL_U32: st = fit<uint32_t>(c, dev_sorted, n, num_leaves, dtype, dev_model, s); break;
+    case TL_U64: st = fit<uint64_t>(c, dev_sorted, n, num_leaves, dtype, dev_model, s); break;
+    case TL_I32: st = fit<int32_t>(c, dev_sorted, n, num_leaves, dtype, dev_model, s); break;
+    case TL_I64: st = fit<int64_t>(c, dev_sorted, n, num_leaves, dtype, dev_model, s); break;
+    default: set_error("unknown dtype"); return TL_EINVAL;
+    }
+    if (st != TL_OK) return st;
+    return tm.stop();
+}
+
+extern "C" tl_status tl_query_btree_learned(const void *dev_A, uint64_t n, uint64_t b,
+                                            tl_dtype dtype, const void *dev_model,
+                                            const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                                            uint64_t *dev_stats, void *stream, float *ms) {
+    if (nq == 0) return TL_OK;
+    if (!dev_A || !dev_model || !dev_q || !dev_ans || n < 2 || b < 2 || b > 64) {
+        set_error("tl_query_btree_learned: need n >= 2, 2 <= b <= 64 and device buffers");
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    unsigned long long *st = (unsigned long long *)dev_stats;
+    tl_status r;
+    switch (dtype) {
+    case TL_U32: r = query<uint32_t>(c, dev_A, n, b, dev_model, dev_q, nq, dev_ans, st, s); break;
+    case TL_U64: r = query<uint64_t>(c, dev_A, n, b, dev_model, dev_q, nq, dev_ans, st, s); break;
+    case TL_I32: r = query<int32_t>(c, dev_A, n, b, dev_model, dev_q, nq, dev_ans, st, s); break;
+    case TL_I64: r = query<int64_t>(c, dev_A, n, b, dev_model, dev_q, nq, dev_ans, st, s); break;
+    default: set_error("unknown dtype"); return TL_EINVAL;
+    }
+    if (r != TL_OK) return r;
+    return tm.stop();
+}

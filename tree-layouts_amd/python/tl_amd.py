@@ -65,7 +65,16 @@ _SIGS = {
                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                        ctypes.POINTER(ctypes.c_uint64)],
 }
-_OPTIONAL = {}
+_OPTIONAL = {
+    "tl_learned_model_bytes": ([ctypes.c_uint32], ctypes.c_uint64),
+    "tl_learned_fit": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
+                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+                       ctypes.c_int),
+    "tl_query_btree_learned": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+}
 
 
 def lib():
@@ -208,3 +217,54 @@ def plan_export(layout, n, b=1, kind="u64", lds_bytes=0):
                             buf.ctypes.data_as(ctypes.c_void_p), used.value,
                             ctypes.byref(used)))
     return buf
+
+
+# ---- learned start node ------------------------------------------------------
+
+def learned_fit(sorted_keys, num_leaves=1 << 16, kind=None, stream=None, timed=True):
+    """Fits the rank model on the sorted keys; returns (model tensor, ms)."""
+    import torch
+    k = _kind(sorted_keys, kind)
+    nbytes = lib().tl_learned_model_bytes(num_leaves)
+    model = torch.zeros((nbytes + 7) // 8, dtype=torch.int64, device=sorted_keys.device)
+    ms = _timed_call(lib().tl_learned_fit, _ptr(sorted_keys), sorted_keys.numel(), DTYPE[k],
+                     num_leaves, _ptr(model), _stream(stream), timed=timed)
+    return model, ms
+
+
+def learned_max_err(model):
+    """largest leaf rank error (uint32 at byte offset 32 of the model)"""
+    return int(model[4].item()) & 0xFFFFFFFF
+
+
+def query_btree_learned(A, b, model, Q, ans, stats=None, kind=None, stream=None, timed=True):
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_btree_learned, _ptr(A), A.numel(), b, DTYPE[k], _ptr(model),
+                       _ptr(Q), Q.numel(), _ptr(ans), _ptr(stats), _stream(stream), timed=timed)
+
+
+def bench_learned(A, keys, Q, plain_ans, b, steps, warmup, n, num_leaves=1 << 16):
+    """Fit on the sorted keys, time the learned queries, compare with plain."""
+    import torch
+    stream = torch.cuda.current_stream()
+    model, fit_ms = learned_fit(keys, num_leaves, kind="u64")
+    ans = torch.empty_like(plain_ans)
+    stats = torch.zeros(1, dtype=torch.int64, device=A.device)
+    query_btree_learned(A, b, model, Q, ans, stats, kind="u64")
+    torch.cuda.synchronize()
+    same = bool(torch.equal(ans, plain_ans))
+    fallbacks = int(stats.item())
+    for _ in range(warmup):
+        query_btree_learned(A, b, model, Q, ans, None, kind="u64", timed=False)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(steps):
+        query_btree_learned(A, b, model, Q, ans, None, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    nq = Q.numel()
+    return {"ms": round(ms, 4), "Mq_s": round(nq / ms / 1e3, 2), "fit_ms": round(fit_ms, 3),
+            "max_rank_err": learned_max_err(model), "fallback_rate": fallbacks / nq,
+            "answers_equal_plain": same, "num_leaves": num_leaves}
