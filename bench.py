@@ -87,6 +87,115 @@ def cpu_baseline(keys_np, b, nq_sample, rng):
             "cores": cores, "n": n, "nq": nq_sample}
 
 
+def run_batch(args, dev):
+    """Uneven batch: 4096 sorted arrays, sizes floor(2^U), U ~ U[8, 22]
+    (seed 5), int32 / uint64 keys 50/50, layouts BST / B-tree b=8 / mvEB
+    round-robin, keys = cumsum of U{1..64} gaps seeded per array, 4 * size
+    uniform-hit queries per array.  One persistent launch (autotuned, graph
+    replay) against one permute + one query call per array."""
+    import torch
+    import tl_amd
+    rng = np.random.default_rng(5)
+    count = args.batch_arrays
+    sizes = np.floor(2.0 ** rng.uniform(8, 22, size=count)).astype(np.int64)
+    kinds = np.where(rng.random(count) < 0.5, "i32", "u64")
+    layouts = ["bst", "btree", "mveb"]
+    arrays, pristine = [], []
+    keys_total = 0
+    perm_bytes = 0.0
+    query_bytes = 0.0
+    log(f"[bench] generating {count} arrays, {int(sizes.sum())} keys")
+    for i in range(count):
+        n, kind, lay = int(sizes[i]), str(kinds[i]), layouts[i % 3]
+        g = torch.Generator(device=dev)
+        g.manual_seed(5 * 1_000_003 + i)
+        tdt = torch.int32 if kind == "i32" else torch.int64
+        gaps = torch.randint(1, 65, (n,), generator=g, device=dev, dtype=torch.int64)
+        keys = torch.cumsum(gaps, 0).to(tdt)
+        qi = torch.randint(0, n, (4 * n,), generator=g, device=dev, dtype=torch.int64)
+        Q = keys[qi]
+        del gaps, qi
+        w = 4 if kind == "i32" else 8
+        arrays.append({"A": keys.clone(), "kind": kind, "layout": lay, "b": 8, "Q": Q,
+                       "ans": torch.empty(4 * n, dtype=torch.int64, device=dev)})
+        pristine.append(keys)
+        keys_total += n
+        perm_bytes += 2.0 * n * w
+        if lay == "btree":
+            depth = 1
+            cap = 8
+            while cap < n:
+                cap = cap * 9 + 8
+                depth += 1
+            node = 8 * w
+        else:
+            depth = int(np.floor(np.log2(n))) + 1
+            node = w
+        query_bytes += 4.0 * n * (w + 8 + depth * node)
+    torch.cuda.synchronize()
+    log("[bench] autotuning on a 256-array sample")
+    cfg = tl_amd.batch_autotune(arrays, sample=256)
+    log(f"[bench] autotuned {cfg}")
+    B = tl_amd.Batch(arrays, lds_bytes=cfg["lds_bytes"], wg_per_cu=cfg["wg_per_cu"])
+    stream = torch.cuda.current_stream()
+
+    def restore():
+        for a, p in zip(arrays, pristine):
+            a["A"].copy_(p)
+        torch.cuda.synchronize()
+
+    # correctness on a sample of arrays after one graph run
+    restore()
+    B.run(graph=True)
+    torch.cuda.synchronize()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    ok = True
+    for i in range(0, count, max(1, count // 48)):
+        a = arrays[i]
+        keys = pristine[i].cpu().numpy()
+        if a["kind"] == "u64":
+            keys = keys.view(np.uint64)
+        exp = ol.permute(a["layout"], keys.copy(), 8)
+        got = a["A"].cpu().numpy().view(keys.dtype)
+        ok &= bool(np.array_equal(got, exp))
+        q = a["Q"].cpu().numpy().view(keys.dtype)
+        ok &= bool(np.array_equal(a["ans"].cpu().numpy().view(np.uint64),
+                                  ol.search(a["layout"], exp, q, a["kind"], 8)))
+    log(f"[bench] batch sample check: {ok}")
+    times = []
+    for step in range(args.warmup + args.steps):
+        restore()
+        t = B.run(graph=True)
+        if step >= args.warmup:
+            times.append(t)
+    restore()
+    t_sep = B.run_separate()
+    ms = float(np.mean(times))
+    total_bytes = perm_bytes + query_bytes
+    achieved = total_bytes / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
+        "value": round(keys_total / (ms * 1e-3) / 1e9, 3), "unit": "Gkeys/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i32/u64",
+        "data": "synthetic: U{1..64} gaps per array (torch CUDA generator), sizes seed 5",
+        "config": {"workload": f"uneven batch: {count} arrays, sizes floor(2^U) U~U[8,22], "
+                               "BST / B-tree b=8 / mvEB round robin, 4*size hit queries each, "
+                               "one persistent launch replayed from a HIP graph",
+                   "keys": keys_total, "queries": 4 * keys_total, "autotuned": cfg},
+        "queries": {"Mq_s": round(4 * keys_total / (ms * 1e-3) / 1e6, 1)},
+        "separate_launches": {"ms": round(t_sep, 3), "speedup": round(t_sep / ms, 2)},
+        "checks": {"sample_arrays": ok},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batch (whole persistent launch, permute + queries)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    B.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +206,8 @@ def main():
     ap.add_argument("--b", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--workload", choices=["btree", "batch"], default="btree")
+    ap.add_argument("--batch-arrays", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -106,6 +217,8 @@ def main():
     import torch
     import tl_amd
     dev = torch.device("cuda:0")
+    if args.workload == "batch":
+        return run_batch(args, dev)
     n, nq, b = 1 << args.log2n, 1 << args.log2q, args.b
     rng = np.random.default_rng(4)
     log(f"[bench] generating n={n} keys and {nq} queries")

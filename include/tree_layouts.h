@@ -121,6 +121,48 @@ tl_status tl_query_btree_learned(const void *dev_A, uint64_t n, uint64_t b, tl_d
                                  const void *dev_model, const void *dev_q, uint64_t nq,
                                  uint64_t *dev_ans, uint64_t *dev_stats, void *stream, float *ms);
 
+/* ---- uneven batch (no counterpart in the reference) --------------------- */
+
+/* One independent array of a batch: n sorted keys at A (device) to be laid
+ * out in place as a BST (layout 0), a B-tree with b keys per node (layout 1,
+ * 1 <= b <= 64) or an mvEB (layout 2), then nq queries Q answered into ans
+ * (device).  n < 2^32. */
+typedef struct {
+    void *A;
+    uint64_t n;
+    tl_dtype dtype;
+    uint32_t layout;
+    uint64_t b;
+    const void *Q;
+    uint64_t nq;
+    uint64_t *ans;
+} tl_batch_item;
+
+/* lds_bytes: LDS budget of the in-LDS sub-layouts (0 = 64 KiB);
+ * wg_per_cu: persistent workgroups per compute unit (0 = 4). */
+typedef struct {
+    uint32_t lds_bytes;
+    uint32_t wg_per_cu;
+    uint32_t reserved;
+} tl_batch_config;
+
+/* Plans every array (the same plans as the single-array permutations) and
+ * uploads the task list; cfg may be NULL. */
+tl_status tl_batch_create(const tl_batch_item *items, uint32_t count, const tl_batch_config *cfg,
+                          void **handle);
+/* Permutes and queries every array of the batch in one persistent launch
+ * (use_graph != 0: captured once in a HIP graph and replayed). */
+tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms);
+/* The same work through one tl_permute_* and one tl_query_* call per array. */
+tl_status tl_batch_run_separate(void *handle, void *stream, float *ms);
+/* Launch plan size: number of ops, of sub-tasks and the LDS bytes per workgroup. */
+tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks, uint64_t *smem_bytes);
+/* Times candidate configurations on device copies of up to `sample` items of
+ * the batch and returns the fastest in *best. */
+tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
+                            void *stream, tl_batch_config *best, float *best_ms);
+tl_status tl_batch_destroy(void *handle);
+
 /* ---- plan inspection (no device needed) -------------------------------- */
 
 /* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2): builds and bounds-

@@ -1,0 +1,94 @@
+"""Uneven batch: every array of a mixed batch (sizes spread over four orders
+of magnitude, int32/uint64 keys, BST / B-tree b=8 / mvEB) comes out of the
+single persistent launch with the oracle's layout and the oracle's answers;
+the graph replay, the plain launch and the per-array path agree."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+import tl_amd
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = ["bst", "btree", "mveb"]
+
+
+def make_batch(count, lo, hi, seed, dev, qmult=4):
+    rng = np.random.default_rng(seed)
+    arrays, host = [], []
+    for i in range(count):
+        n = int(2 ** rng.uniform(lo, hi))
+        kind = "i32" if rng.random() < 0.5 else "u64"
+        layout = LAYOUTS[i % 3]
+        gaps = rng.integers(1, 65, size=n)
+        keys = np.cumsum(gaps).astype(np.int32 if kind == "i32" else np.uint64)
+        if kind == "i32":
+            keys = (keys.astype(np.int64) - int(keys[n // 2])).astype(np.int32)
+        nq = qmult * n
+        q = keys[rng.integers(0, n, size=nq)]
+        miss = rng.integers(int(keys.min()) - 3, int(keys.max()) + 3, size=nq // 4)
+        q = np.concatenate([q, miss.astype(keys.dtype)])
+        st = np.int32 if kind == "i32" else np.int64
+        A = torch.from_numpy(keys.view(st).copy()).to(dev)
+        Q = torch.from_numpy(q.view(st).copy()).to(dev)
+        ans = torch.full((q.size,), -1, dtype=torch.int64, device=dev)
+        arrays.append({"A": A, "kind": kind, "layout": layout, "b": 8, "Q": Q, "ans": ans})
+        host.append((keys, q))
+    return arrays, host
+
+
+def check(arrays, host):
+    for a, (keys, q) in zip(arrays, host):
+        exp_layout = ol.permute(a["layout"], keys.copy(), 8)
+        got = a["A"].cpu().numpy().view(keys.dtype)
+        assert np.array_equal(got, exp_layout), (a["layout"], a["kind"], keys.size)
+        exp_ans = ol.search(a["layout"], exp_layout, q, a["kind"], 8)
+        assert np.array_equal(a["ans"].cpu().numpy().view(np.uint64), exp_ans), \
+            (a["layout"], a["kind"], keys.size)
+
+
+def restore(arrays, host, dev):
+    for a, (keys, _) in zip(arrays, host):
+        st = np.int32 if keys.dtype.itemsize == 4 else np.int64
+        a["A"].copy_(torch.from_numpy(keys.view(st).copy()).to(dev))
+        a["ans"].fill_(-1)
+
+
+@pytest.mark.parametrize("lds", [0, 32768])
+def test_batch_matches_oracle(lds, dev):
+    arrays, host = make_batch(240, 0, 18, 5, dev)
+    B = tl_amd.Batch(arrays, lds_bytes=lds)
+    st = B.stats()
+    assert st["tasks"] > st["ops"] > 240
+    B.run(graph=False)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    # graph capture and two replays, with the arrays restored in between
+    for _ in range(2):
+        restore(arrays, host, dev)
+        torch.cuda.synchronize()
+        B.run(graph=True)
+        torch.cuda.synchronize()
+        check(arrays, host)
+    B.close()
+
+
+def test_batch_separate_path_matches(dev):
+    arrays, host = make_batch(60, 0, 16, 6, dev)
+    B = tl_amd.Batch(arrays)
+    B.run_separate()
+    torch.cuda.synchronize()
+    check(arrays, host)
+    B.close()
+
+
+def test_batch_autotune_returns_a_config(dev):
+    arrays, host = make_batch(90, 6, 16, 7, dev)
+    cfg = tl_amd.batch_autotune(arrays, sample=30)
+    assert cfg["lds_bytes"] in (32768, 65536, 98304) and cfg["wg_per_cu"] in (2, 4, 8)
+    B = tl_amd.Batch(arrays, lds_bytes=cfg["lds_bytes"], wg_per_cu=cfg["wg_per_cu"])
+    B.run(graph=True)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    B.close()

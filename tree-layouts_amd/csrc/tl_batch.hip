@@ -1,1 +1,917 @@
-// placeholder
+// tl_batch.hip -- uneven-batch engine: permute and query thousands of
+// independent sorted arrays of different sizes, key types and layouts in one
+// persistent launch (optionally captured in a HIP graph).
+//
+// Every array gets its ordinary permutation plan (tl_plan.hpp; the LDS budget
+// comes from the batch configuration) and a chain of query chunks.  Each
+// launch of a plan becomes an "op" with a number of independent sub-tasks
+// (items, tiles, cycle x column chunks, chained moves, query chunks).  All
+// sub-tasks of all arrays go into one global list ordered by (depth of the op
+// in its array's chain, array, op, sub-task).  Workgroups of one persistent
+// kernel take tickets from an atomic counter, wait until the previous op of
+// the same array has completed (its completion counter), run the sub-task and
+// count it done.  A sub-task only waits for sub-tasks with smaller tickets,
+// which already run in workgroups that never wait for larger tickets, so the
+// launch cannot deadlock; every wait is still bounded and reports a timeout
+// through the error word instead of hanging.
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "tl_internal.hpp"
+#include "tl_kernels.hpp"
+#include "tl_plan.hpp"
+
+namespace tl {
+
+enum : uint32_t { OP_QUERY = 100 };
+constexpr uint32_t BATCH_THREADS = 256;
+
+struct BOp {
+    uint32_t type, width, dtype, layout;
+    uint64_t A;                       // array base (device address)
+    uint64_t runs_at, nruns, nitems;  // runs in the blob
+    uint64_t p[8];
+    MoveSpec ms;
+    uint64_t ntasks;
+    int64_t prev;                     // op that must complete first, -1 if none
+    uint64_t flags_at;                // chained moves: first flag word
+    uint64_t stash_at;                // copy ops: element offset in the stash area
+    // queries
+    uint64_t Q, nq, ans, n, b;
+    uint64_t tbl_at;                  // mvEB: tables (3 words each) in the blob
+    uint64_t idx_at;                  // mvEB: idx (nt words) then table_idx (nt words)
+    uint32_t ntables, tsize;
+};
+
+struct BTask {
+    uint32_t op, pad;
+    uint64_t sub;
+};
+
+// ---------------------------------------------------------------------------
+// device sub-task executors (each mirrors one single-array kernel)
+
+__device__ inline uint64_t b_run_offset(const uint64_t *runs, uint32_t nruns, uint64_t idx) {
+    uint32_t lo = 0, hi = nruns;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (runs[4 * mid + 3] <= idx)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return runs[4 * lo] + (idx - runs[4 * lo + 3]) * runs[4 * lo + 1];
+}
+
+template <typename T>
+__device__ void ex_lds(const BOp &op, uint64_t sub, const uint64_t *blob, unsigned char *smem) {
+    T *A = (T *)op.A;
+    T *s = (T *)smem;
+    const uint64_t *runs = blob + op.runs_at;
+    const uint32_t n = (uint32_t)op.p[0], kind = (uint32_t)op.p[1], b = (uint32_t)op.p[2];
+    const uint32_t rows = (uint32_t)op.p[3], per = (uint32_t)op.p[4];
+    __shared__ uint64_t s_off[64];
+    BtreeShape sh;
+    if (kind == LDS_BTREE) sh = btree_shape(n, b);
+    const uint64_t first = sub * per;
+    const uint32_t cnt = (uint32_t)min((uint64_t)per, op.nitems - first);
+    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
+        s_off[t] = b_run_offset(runs, (uint32_t)op.nruns, first + t);
+    __syncthreads();
+    const uint32_t tot = cnt * n;
+    for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
+        const uint32_t it = i / n, x = i - it * n;
+        const T v = A[s_off[it] + x];
+        uint32_t p;
+        if (kind == LDS_BTREE)
+            p = btree_pos32(x, sh);
+        else if (kind == LDS_MVEB)
+            p = mveb_pos32(x, n);
+        else if (kind == LDS_GATHER)
+            p = gather_pos32(x, rows, b);
+        else {
+            const uint32_t y = x + 1, t = 31u - __clz(y);
+            p = ((1u << t) | rev_low_bits32(y & ((1u << t) - 1), t)) - 1;
+        }
+        s[it * n + p] = v;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
+        const uint32_t t = i / n, x = i - t * n;
+        A[s_off[t] + x] = s[i];
+    }
+}
+
+constexpr uint64_t REV_CHUNK = BATCH_THREADS * 8;
+
+template <typename T>
+__device__ void ex_reverse(const BOp &op, uint64_t sub, const uint64_t *blob) {
+    T *A = (T *)op.A;
+    const uint64_t len = op.p[0], half = len / 2;
+    const uint64_t per = (half + REV_CHUNK - 1) / REV_CHUNK;
+    const uint64_t seg = sub / per, chunk = sub % per;
+    const uint64_t off = b_run_offset(blob + op.runs_at, (uint32_t)op.nruns, seg);
+    const uint64_t lo = chunk * REV_CHUNK, hi = min(half, lo + REV_CHUNK);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const T x = A[off + i];
+        const T y = A[off + len - 1 - i];
+        A[off + i] = y;
+        A[off + len - 1 - i] = x;
+    }
+}
+
+template <typename T, int K>
+__device__ void ex_bitrev(const BOp &op, uint64_t m, unsigned char *smem) {
+    constexpr int R = 1 << K;
+    T(*sA)[R + 1] = (T(*)[R + 1])smem;
+    T(*sB)[R + 1] = (T(*)[R + 1])(smem + sizeof(T) * R * (R + 1));
+    T *A = (T *)op.A;
+    const int64_t base = (int64_t)op.p[0];
+    const uint32_t D = (uint32_t)op.p[1];
+    const uint32_t MB = D - 2 * K;
+    const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
+    if (m > mr) return;
+    const uint32_t hs = D - K;
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const uint32_t h = e >> K, l = e & (R - 1);
+        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
+        if (ia >= 0) sA[h][l] = A[ia];
+        if (m != mr) sB[h][l] = A[base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l)];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const uint32_t h = e >> K, l = e & (R - 1);
+        const uint32_t hr = rev_low_bits32(l, K), lr = rev_low_bits32(h, K);
+        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
+        if (m != mr) {
+            A[ia] = sB[hr][lr];
+            A[base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l)] = sA[hr][lr];
+        } else if (ia >= 0) {
+            A[ia] = sA[hr][lr];
+        }
+    }
+}
+
+template <typename T>
+__device__ void ex_unit_merge(const BOp &op, uint64_t sub, const uint64_t *blob) {
+    T *A = (T *)op.A;
+    const uint64_t unit = op.p[0], ncyc = op.p[2];
+    const uint64_t nchunks = (unit + BATCH_THREADS - 1) / BATCH_THREADS;
+    const uint64_t chunk = sub % nchunks;
+    const uint64_t cyc = (sub / nchunks) % ncyc;
+    const uint64_t parent = sub / (nchunks * ncyc);
+    const uint64_t c = chunk * BATCH_THREADS + threadIdx.x;
+    if (c >= unit) return;
+    const uint64_t *cstart = blob + op.p[1];
+    const uint64_t *u = blob + op.p[3] + cstart[cyc];
+    const int64_t L = (int64_t)(cstart[cyc + 1] - cstart[cyc]);
+    T *a = A + b_run_offset(blob + op.runs_at, (uint32_t)op.nruns, parent) + c;
+    T hold = a[u[L - 1] * unit];
+    int64_t k = L - 2;
+    while (k >= 0) {
+        T v[8];
+        const int nb = k + 1 < 8 ? (int)(k + 1) : 8;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < nb) v[j] = a[u[k - j] * unit];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < nb) a[u[k - j + 1] * unit] = v[j];
+        k -= nb;
+    }
+    a[u[0] * unit] = hold;
+}
+
+template <typename T>
+__device__ void ex_sparse_stream(const BOp &op, uint64_t item, const uint64_t *blob,
+                                 unsigned char *smem) {
+    T *roots = (T *)smem;
+    T *a = (T *)op.A + b_run_offset(blob + op.runs_at, (uint32_t)op.nruns, item);
+    const uint64_t r = op.p[0], l = op.p[1], L1 = l + 1, total = r * L1;
+    constexpr int U = 8;
+    const uint64_t batch = (uint64_t)blockDim.x * U;
+    uint64_t hi = total;
+    while (hi > 0) {
+        const uint64_t lo = hi > batch ? hi - batch : 0;
+        T v[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint64_t x = lo + threadIdx.x + (uint64_t)k * blockDim.x;
+            if (x < hi) v[k] = a[x];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint64_t x = lo + threadIdx.x + (uint64_t)k * blockDim.x;
+            if (x < hi) {
+                const uint64_t row = x / L1, cc = x - row * L1;
+                if (cc == l)
+                    roots[row] = v[k];
+                else
+                    a[r + row * l + cc] = v[k];
+            }
+        }
+        hi = lo;
+    }
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < r; i += blockDim.x) a[i] = roots[i];
+}
+
+constexpr uint64_t COPY_CHUNK = BATCH_THREADS * 8;
+
+template <typename T>
+__device__ void ex_copy(const BOp &op, uint64_t sub, T *stash) {
+    T *A = (T *)op.A;
+    const uint64_t P = op.p[0], stride = op.p[1], count = op.p[2];
+    T *st = stash + op.stash_at;
+    const uint64_t lo = sub * COPY_CHUNK, hi = min(count, lo + COPY_CHUNK);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        if (op.p[3])
+            st[i] = A[P + i * stride];
+        else
+            A[P + i * stride] = st[i];
+    }
+}
+
+constexpr int MOVE_PER = 16;
+
+template <typename T>
+__device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32_t *err) {
+    T *A = (T *)op.A;
+    uint32_t *flags = flagbase + op.flags_at;
+    uint64_t src, dst, len;
+    move_at(op.ms, t, src, dst, len);
+    T v[MOVE_PER];
+#pragma unroll
+    for (int j = 0; j < MOVE_PER; j++) {
+        const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+        if (idx < len) v[j] = A[src + idx];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        wait_deps(op.ms, t, dst, len, flags, err);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MOVE_PER; j++) {
+        const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+        if (idx < len) A[dst + idx] = v[j];
+    }
+}
+
+// ---- queries: QCHUNK queries per sub-task --------------------------------
+constexpr uint64_t QCHUNK = 4096;
+
+template <typename K>
+__device__ void ex_query_bst(const BOp &op, uint64_t sub) {
+    const K *A = (const K *)op.A, *Q = (const K *)op.Q;
+    uint64_t *ans = (uint64_t *)op.ans;
+    const uint64_t n = op.n;
+    const uint64_t lo = sub * QCHUNK, hi = min(op.nq, lo + QCHUNK);
+    for (uint64_t qi = lo + threadIdx.x; qi < hi; qi += blockDim.x) {
+        const K q = Q[qi];
+        uint64_t i = 0, res = n;
+        while (i < n) {
+            const K cur = A[i];
+            if (q == cur) {
+                res = i;
+                break;
+            }
+            i = 2 * i + 1 + (q > cur);
+        }
+        ans[qi] = res;
+    }
+}
+
+template <typename K, int G>
+__device__ void ex_query_btree(const BOp &op, uint64_t sub) {
+    const K *A = (const K *)op.A, *Q = (const K *)op.Q;
+    uint64_t *ans = (uint64_t *)op.ans;
+    const uint64_t n = op.n;
+    const uint32_t b = (uint32_t)op.b;
+    const uint32_t lane = threadIdx.x & 63, g = lane / G, j = lane % G;
+    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1);
+    const uint64_t per_pass = blockDim.x / G;
+    const uint64_t lo = sub * QCHUNK, hi = min(op.nq, lo + QCHUNK);
+    for (uint64_t base = lo; base < hi; base += per_pass) {
+        const uint64_t qi = base + threadIdx.x / G;
+        const bool valid = qi < hi;
+        K q{};
+        if (valid) q = Q[qi];
+        uint64_t i = 0, res = n;
+        bool active = valid;
+        while (__any(active)) {
+            bool lt = false, eq = false;
+            if (active && j < b && i + j < n) {
+                const K key = A[i + j];
+                lt = key < q;
+                eq = key == q;
+            }
+            const uint64_t mlt = (__ballot(lt) >> (g * G)) & gmask;
+            const uint64_t meq = (__ballot(eq) >> (g * G)) & gmask;
+            if (active) {
+                const uint32_t js = (uint32_t)__popcll(mlt);
+                if ((meq >> js) & 1ull) {
+                    res = i + js;
+                    active = false;
+                } else {
+                    i = (i + 1) * (b + 1) + (uint64_t)js * b - 1;
+                    if (i >= n) active = false;
+                }
+            }
+        }
+        if (valid && j == 0) ans[qi] = res;
+    }
+}
+
+template <typename K>
+__device__ void ex_query_mveb(const BOp &op, uint64_t sub, const uint64_t *blob,
+                              unsigned char *smem) {
+    const K *A = (const K *)op.A, *Q = (const K *)op.Q;
+    uint64_t *ans = (uint64_t *)op.ans;
+    const uint64_t n = op.n;
+    uint32_t *pos = (uint32_t *)smem;   // [depth][lane], n < 2^32 in a batch
+    const uint64_t *tb = blob + op.tbl_at;
+    const uint64_t *idx = blob + op.idx_at;
+    const uint64_t *tidx = idx + op.ntables;
+    const uint32_t me = threadIdx.x, W = blockDim.x;
+    const uint64_t lo = sub * QCHUNK, hi = min(op.nq, lo + QCHUNK);
+    for (uint64_t qi = lo + threadIdx.x; qi < hi; qi += blockDim.x) {
+        const K q = Q[qi];
+        uint64_t res = n, offset = 0;
+        for (uint32_t jj = 0; jj < op.ntables; jj++) {
+            const uint64_t *t = tb + 3 * tidx[jj];
+            const uint32_t dj = floor_log2_64(n - offset) + 1;
+            const uint64_t end = idx[jj];
+            uint64_t bfs = 1, index = 0;
+            uint32_t cd = 0;
+            pos[me] = 1;
+            bool done = false;
+            while (index + offset < end) {
+                const K cur = A[offset + index];
+                if (q == cur) {
+                    res = offset + index;
+                    done = true;
+                    break;
+                }
+                bfs = 2 * bfs + (q > cur);
+                cd++;
+                if (cd >= dj) {
+                    done = true;
+                    break;
+                }
+                const uint64_t L = t[3 * cd], R = t[3 * cd + 1];
+                const uint32_t D = (uint32_t)t[3 * cd + 2];
+                const uint64_t p = (uint64_t)pos[D * W + me] + R + (bfs & R) * L;
+                pos[cd * W + me] = (uint32_t)p;
+                index = p - 1;
+            }
+            if (done) break;
+            offset = end;
+        }
+        ans[qi] = res;
+    }
+}
+
+template <typename K>
+__device__ void ex_query(const BOp &op, uint64_t sub, const uint64_t *blob, unsigned char *smem) {
+    if (op.layout == 0) {
+        ex_query_bst<K>(op, sub);
+    } else if (op.layout == 1) {
+        const uint64_t b = op.b;
+        if (b <= 2)
+            ex_query_btree<K, 2>(op, sub);
+        else if (b <= 4)
+            ex_query_btree<K, 4>(op, sub);
+        else if (b <= 8)
+            ex_query_btree<K, 8>(op, sub);
+        else if (b <= 16)
+            ex_query_btree<K, 16>(op, sub);
+        else if (b <= 32)
+            ex_query_btree<K, 32>(op, sub);
+        else
+            ex_query_btree<K, 64>(op, sub);
+    } else {
+        ex_query_mveb<K>(op, sub, blob, smem);
+    }
+}
+
+template <typename T>
+__device__ void ex_permute_op(const BOp &op, uint64_t sub, const uint64_t *blob,
+                              unsigned char *smem, T *stash, uint32_t *flags, uint32_t *err) {
+    switch (op.type) {
+    case OP_LDS: ex_lds<T>(op, sub, blob, smem); break;
+    case OP_REVERSE: ex_reverse<T>(op, sub, blob); break;
+    case OP_BITREV:
+        if (op.p[2] == 6)
+            ex_bitrev<T, 6>(op, sub, smem);
+        else
+            ex_bitrev<T, 5>(op, sub, smem);
+        break;
+    case OP_UNIT_MERGE: ex_unit_merge<T>(op, sub, blob); break;
+    case OP_SPARSE_STREAM: ex_sparse_stream<T>(op, sub, blob, smem); break;
+    case OP_COPY: ex_copy<T>(op, sub, stash); break;
+    case OP_CHAINED: ex_chained<T>(op, sub, flags, err); break;
+    default: break;
+    }
+}
+
+// bounded wait for an op's completion counter
+__device__ inline bool wait_done(unsigned long long *done, uint64_t want, uint32_t *err) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 24)) {
+            atomicOr(err, 2u);
+            return false;
+        }
+        if ((spins & 255u) == 0u &&
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(BATCH_THREADS) void k_batch(const BOp *__restrict__ ops,
+                                                          const BTask *__restrict__ tasks,
+                                                          uint64_t ntasks, const uint64_t *blob,
+                                                          unsigned long long *done,
+                                                          unsigned long long *ticket,
+                                                          uint32_t *flags, void *stash,
+                                                          uint32_t *err) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint64_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        __syncthreads();
+        if (t >= ntasks) return;
+        const BTask tk = tasks[t];
+        const BOp &op = ops[tk.op];
+        if (threadIdx.x == 0 && op.prev >= 0) wait_done(&done[op.prev], ops[op.prev].ntasks, err);
+        __syncthreads();
+        if (op.type == OP_QUERY) {
+            switch (op.dtype) {
+            case TL_U32: ex_query<uint32_t>(op, tk.sub, blob, smem); break;
+            case TL_U64: ex_query<uint64_t>(op, tk.sub, blob, smem); break;
+            case TL_I32: ex_query<int32_t>(op, tk.sub, blob, smem); break;
+            default: ex_query<int64_t>(op, tk.sub, blob, smem); break;
+            }
+        } else if (op.width == 4) {
+            ex_permute_op<uint32_t>(op, tk.sub, blob, smem, (uint32_t *)stash, flags, err);
+        } else {
+            ex_permute_op<uint64_t>(op, tk.sub, blob, smem, (uint64_t *)stash, flags, err);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(&done[tk.op], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// zero the completion counters, the chained-move flags, the ticket and the
+// error word with a kernel (graph-safe; no memset nodes)
+__global__ void k_batch_clear(unsigned long long *done, uint64_t nops, uint32_t *flags,
+                              uint64_t nflags, unsigned long long *ticket, uint32_t *err) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = tid; i < nops; i += stride) done[i] = 0;
+    for (uint64_t i = tid; i < nflags; i += stride) flags[i] = 0;
+    if (tid == 0) {
+        *ticket = 0;
+        *err = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct Batch {
+    std::vector<tl_batch_item> items;
+    tl_batch_config cfg{};
+    std::vector<BOp> ops;
+    std::vector<BTask> tasks;
+    std::vector<uint64_t> blob;
+    uint64_t nflags = 0, stash_bytes = 0;
+    // device
+    BOp *d_ops = nullptr;
+    BTask *d_tasks = nullptr;
+    uint64_t *d_blob = nullptr;
+    unsigned long long *d_done = nullptr, *d_ticket = nullptr;
+    uint32_t *d_flags = nullptr, *d_err = nullptr;
+    void *d_stash = nullptr;
+    size_t smem = 0;
+    uint32_t grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~Batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : {(void *)d_ops, (void *)d_tasks, (void *)d_blob, (void *)d_done,
+                        (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash})
+            if (p) (void)hipFree(p);
+    }
+};
+
+static uint64_t op_tasks(const Op &op, uint32_t W) {
+    switch (op.type) {
+    case OP_LDS: return (op.nitems + op.p[4] - 1) / op.p[4];
+    case OP_REVERSE: {
+        const uint64_t half = op.p[0] / 2;
+        return half ? op.nitems * ((half + REV_CHUNK - 1) / REV_CHUNK) : 0;
+    }
+    case OP_BITREV: return 1ull << (op.p[1] - 2 * op.p[2]);
+    case OP_UNIT_MERGE:
+        return op.nitems * op.p[2] * ((op.p[0] + BATCH_THREADS - 1) / BATCH_THREADS);
+    case OP_SPARSE_STREAM: return op.nitems;
+    case OP_COPY: return (op.p[2] + COPY_CHUNK - 1) / COPY_CHUNK;
+    case OP_CHAINED: return op.p[0];
+    }
+    (void)W;
+    return 0;
+}
+
+static void host_tableset(uint64_t n, std::vector<uint64_t> &tables, std::vector<uint64_t> &idx,
+                          std::vector<uint64_t> &tidx) {
+    const uint32_t d = floor_log2_64(n) + 1;
+    std::vector<uint64_t> sizes{n};
+    std::vector<uint32_t> depths{d};
+    if (n != (1ull << d) - 1) {
+        uint64_t tn = n;
+        uint32_t td = d;
+        while (td >= 2) {
+            uint32_t rd, ld;
+            mveb_split(td, rd, ld);
+            const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
+            const uint64_t nf = (tn - r) / l, inc = tn - r - nf * l;
+            if (inc == 0) break;
+            const uint32_t incd = floor_log2_64(inc) + 1;
+            sizes.push_back(inc);
+            depths.push_back(incd);
+            tn = inc;
+            td = incd;
+            if (inc == (1ull << incd) - 1) break;
+        }
+    }
+    uint32_t off = 0;
+    tables.clear();
+    idx.clear();
+    tidx.clear();
+    for (size_t j = 0; j < sizes.size(); j++) {
+        std::vector<tl_veb_table> t(depths[j], tl_veb_table{0, 0, 0});
+        tl_veb_build_table(t.data(), sizes[j], depths[j], 0);
+        for (auto &e : t) {
+            tables.push_back(e.L);
+            tables.push_back(e.R);
+            tables.push_back(e.D);
+        }
+        tidx.push_back(off);
+        off += depths[j];
+        idx.push_back(j + 1 < sizes.size() ? n - sizes[j + 1] : n);
+    }
+}
+
+static tl_status build_batch(Batch &B, Ctx *c) {
+    B.ops.clear();
+    B.tasks.clear();
+    B.blob.clear();
+    B.nflags = 0;
+    uint64_t stash_elems = 0;
+    Tuning tu;
+    tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max);
+    struct Pending {
+        uint32_t op, depth, array;
+    };
+    std::vector<Pending> order;
+    uint64_t max_smem = 0;
+    for (uint32_t a = 0; a < B.items.size(); a++) {
+        const tl_batch_item &it = B.items[a];
+        uint32_t W;
+        if (!width_of(it.dtype, W) || it.layout > 2 || (it.layout == 1 && (it.b < 1 || it.b > 64)) ||
+            it.n >= (1ull << 32) || (it.n > 1 && !it.A) || (it.nq && (!it.Q || !it.ans))) {
+            set_error("tl_batch: bad item " + std::to_string(a));
+            return TL_EINVAL;
+        }
+        Plan pl;
+        const uint64_t b = it.layout == 0 ? 1 : it.b;
+        build_plan(pl, tu, (Layout)it.layout, it.n, b, W);
+        std::string why;
+        if (!validate_plan(pl, it.n, why)) {
+            set_error(why);
+            return TL_EINVAL;
+        }
+        const uint64_t blob_base = B.blob.size();
+        B.blob.insert(B.blob.end(), pl.blob.begin(), pl.blob.end());
+        const uint64_t stash_at = stash_elems;
+        stash_elems += pl.max_stash;
+        int64_t prev = -1;
+        uint32_t depth = 0;
+        for (const Op &op : pl.ops) {
+            BOp bo{};
+            bo.type = op.type;
+            bo.width = W;
+            bo.dtype = it.dtype;
+            bo.layout = it.layout;
+            bo.A = (uint64_t)it.A;
+            bo.runs_at = op.runs_at + blob_base;
+            bo.nruns = op.nruns;
+            bo.nitems = op.nitems;
+            for (int k = 0; k < 8; k++) bo.p[k] = op.p[k];
+            if (op.type == OP_UNIT_MERGE) {
+                bo.p[1] += blob_base;
+                bo.p[3] += blob_base;
+            }
+            bo.ms = op.ms;
+            bo.ntasks = op_tasks(op, W);
+            bo.prev = prev;
+            bo.stash_at = stash_at;
+            if (op.type == OP_CHAINED) {
+                bo.flags_at = B.nflags;
+                B.nflags += op.p[0];
+            }
+            if (op.type == OP_LDS) max_smem = std::max(max_smem, op.p[0] * op.p[4] * W);
+            if (op.type == OP_SPARSE_STREAM) max_smem = std::max(max_smem, op.p[0] * W);
+            if (op.type == OP_BITREV) {
+                const uint64_t R = 1ull << op.p[2];
+                max_smem = std::max(max_smem, 2 * R * (R + 1) * W);
+            }
+            if (bo.ntasks == 0) continue;
+            const uint32_t id = (uint32_t)B.ops.size();
+            B.ops.push_back(bo);
+            order.push_back({id, depth++, a});
+            prev = id;
+        }
+        if (it.nq) {
+            BOp q{};
+            q.type = OP_QUERY;
+            q.width = W;
+            q.dtype = it.dtype;
+            q.layout = it.layout;
+            q.A = (uint64_t)it.A;
+            q.Q = (uint64_t)it.Q;
+            q.nq = it.nq;
+            q.ans = (uint64_t)it.ans;
+            q.n = it.n;
+            q.b = b;
+            q.prev = prev;
+            q.ntasks = (it.nq + QCHUNK - 1) / QCHUNK;
+            if (it.layout == 2 && it.n > 0) {
+                std::vector<uint64_t> tb, ix, tix;
+                host_tableset(it.n, tb, ix, tix);
+                q.tbl_at = B.blob.size();
+                B.blob.insert(B.blob.end(), tb.begin(), tb.end());
+                q.idx_at = B.blob.size();
+                B.blob.insert(B.blob.end(), ix.begin(), ix.end());
+                B.blob.insert(B.blob.end(), tix.begin(), tix.end());
+                q.ntables = (uint32_t)ix.size();
+                q.tsize = (uint32_t)(tb.size() / 3);
+                const uint64_t dmax = floor_log2_64(it.n) + 2;
+                max_smem = std::max(max_smem, dmax * BATCH_THREADS * 4);
+            } else if (it.layout == 2) {
+                q.ntables = 0;
+            }
+            const uint32_t id = (uint32_t)B.ops.size();
+            B.ops.push_back(q);
+            order.push_back({id, depth++, a});
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [](const Pending &x, const Pending &y) {
+        return x.depth != y.depth ? x.depth < y.depth : x.array < y.array;
+    });
+    for (const Pending &pd : order)
+        for (uint64_t s = 0; s < B.ops[pd.op].ntasks; s++) B.tasks.push_back({pd.op, 0, s});
+    B.smem = std::max<uint64_t>(max_smem, 1024);
+    if (B.smem > c->lds_max) {
+        set_error("tl_batch: LDS budget exceeds the device limit");
+        return TL_EINVAL;
+    }
+    B.stash_bytes = std::max<uint64_t>(stash_elems * 8, 64);
+    B.grid = (uint32_t)std::max<uint64_t>(1, (uint64_t)c->num_cus * B.cfg.wg_per_cu);
+    return TL_OK;
+}
+
+static tl_status upload_batch(Batch &B) {
+    auto up = [](void **dst, const void *src, size_t bytes) -> tl_status {
+        if (*dst) {
+            (void)hipFree(*dst);
+            *dst = nullptr;
+        }
+        TL_HIP(hipMalloc(dst, std::max<size_t>(bytes, 64)));
+        if (bytes) TL_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return TL_OK;
+    };
+    TL_TRY(up((void **)&B.d_ops, B.ops.data(), B.ops.size() * sizeof(BOp)));
+    TL_TRY(up((void **)&B.d_tasks, B.tasks.data(), B.tasks.size() * sizeof(BTask)));
+    TL_TRY(up((void **)&B.d_blob, B.blob.data(), B.blob.size() * 8));
+    TL_TRY(up((void **)&B.d_done, nullptr, B.ops.size() * 8));
+    TL_TRY(up((void **)&B.d_ticket, nullptr, 64));
+    TL_TRY(up((void **)&B.d_flags, nullptr, B.nflags * 4));
+    TL_TRY(up((void **)&B.d_err, nullptr, 64));
+    TL_TRY(up(&B.d_stash, nullptr, B.stash_bytes));
+    return TL_OK;
+}
+
+static void launch_batch(Batch &B, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_clear, dim3(256), dim3(256), 0, s, B.d_done, (uint64_t)B.ops.size(),
+                       B.d_flags, B.nflags, B.d_ticket, B.d_err);
+    hipLaunchKernelGGL(k_batch, dim3(B.grid), dim3(BATCH_THREADS), B.smem, s, B.d_ops, B.d_tasks,
+                       (uint64_t)B.tasks.size(), B.d_blob, B.d_done, B.d_ticket, B.d_flags,
+                       B.d_stash, B.d_err);
+}
+
+}   // namespace tl
+
+using namespace tl;
+
+extern "C" tl_status tl_batch_create(const tl_batch_item *items, uint32_t count,
+                                     const tl_batch_config *cfg, void **handle) {
+    if (!handle || (count && !items)) {
+        set_error("tl_batch_create: bad arguments");
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    auto B = std::make_unique<Batch>();
+    B->items.assign(items, items + count);
+    B->cfg = cfg ? *cfg : tl_batch_config{65536, 4, 0};
+    if (B->cfg.lds_bytes == 0) B->cfg.lds_bytes = 65536;
+    if (B->cfg.wg_per_cu == 0) B->cfg.wg_per_cu = 4;
+    TL_TRY(build_batch(*B, c));
+    TL_TRY(upload_batch(*B));
+    *handle = B.release();
+    return TL_OK;
+}
+
+extern "C" tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks,
+                                    uint64_t *smem_bytes) {
+    if (!handle) return TL_EINVAL;
+    Batch *B = (Batch *)handle;
+    if (ops) *ops = B->ops.size();
+    if (tasks) *tasks = B->tasks.size();
+    if (smem_bytes) *smem_bytes = B->smem;
+    return TL_OK;
+}
+
+extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms) {
+    if (!handle) {
+        set_error("tl_batch_run: null handle");
+        return TL_EINVAL;
+    }
+    Batch *B = (Batch *)handle;
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    if (use_graph) {
+        if (!B->gexec || B->graph_stream != s) {
+            if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+            if (B->graph) (void)hipGraphDestroy(B->graph);
+            B->gexec = nullptr;
+            B->graph = nullptr;
+            hipStream_t cap;
+            TL_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            TL_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            launch_batch(*B, cap);
+            TL_HIP(hipStreamEndCapture(cap, &B->graph));
+            TL_HIP(hipStreamDestroy(cap));
+            TL_HIP(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+            B->graph_stream = s;
+        }
+        TL_TRY(tm.start(s, ms));
+        TL_HIP(hipGraphLaunch(B->gexec, s));
+    } else {
+        TL_TRY(tm.start(s, ms));
+        launch_batch(*B, s);
+        TL_LAUNCH_CHECK();
+    }
+    TL_TRY(tm.stop());
+    if (ms) {
+        uint32_t e = 0;
+        TL_HIP(hipMemcpy(&e, B->d_err, 4, hipMemcpyDeviceToHost));
+        if (e) {
+            set_error("tl_batch_run: an in-kernel bounded wait timed out");
+            return TL_ETIMEOUT;
+        }
+    }
+    return TL_OK;
+}
+
+// the comparison path: every array through the single-array entry points
+extern "C" tl_status tl_batch_run_separate(void *handle, void *stream, float *ms) {
+    if (!handle) return TL_EINVAL;
+    Batch *B = (Batch *)handle;
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    for (const tl_batch_item &it : B->items) {
+        if (it.layout == 0)
+            TL_TRY(tl_permute_bst(it.A, it.n, it.dtype, stream, nullptr));
+        else if (it.layout == 1)
+            TL_TRY(tl_permute_btree(it.A, it.n, it.b, it.dtype, stream, nullptr));
+        else
+            TL_TRY(tl_permute_mveb(it.A, it.n, it.dtype, stream, nullptr));
+        if (!it.nq) continue;
+        if (it.layout == 0)
+            TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
+        else if (it.layout == 1)
+            TL_TRY(tl_query_btree(it.A, it.n, it.b, it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
+        else {
+            std::vector<uint64_t> tb, ix, tix;
+            host_tableset(it.n, tb, ix, tix);
+            // device copies of the table set, in the reference's packed format
+            std::vector<tl_veb_table> t(tb.size() / 3);
+            for (size_t k = 0; k < t.size(); k++)
+                t[k] = tl_veb_table{tb[3 * k], tb[3 * k + 1], (uint32_t)tb[3 * k + 2]};
+            std::vector<uint32_t> t32(tix.begin(), tix.end());
+            void *dt, *di, *dti;
+            TL_HIP(hipMalloc(&dt, t.size() * sizeof(tl_veb_table)));
+            TL_HIP(hipMalloc(&di, ix.size() * 8));
+            TL_HIP(hipMalloc(&dti, t32.size() * 4));
+            TL_HIP(hipMemcpy(dt, t.data(), t.size() * sizeof(tl_veb_table), hipMemcpyHostToDevice));
+            TL_HIP(hipMemcpy(di, ix.data(), ix.size() * 8, hipMemcpyHostToDevice));
+            TL_HIP(hipMemcpy(dti, t32.data(), t32.size() * 4, hipMemcpyHostToDevice));
+            TL_TRY(tl_query_mveb_nonperfect(it.A, (const tl_veb_table *)dt, (const uint64_t *)di,
+                                            (const uint32_t *)dti, it.n, 0, (uint32_t)ix.size(),
+                                            it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
+            TL_HIP(hipStreamSynchronize(s));
+            (void)hipFree(dt);
+            (void)hipFree(di);
+            (void)hipFree(dti);
+        }
+    }
+    return tm.stop();
+}
+
+extern "C" tl_status tl_batch_destroy(void *handle) {
+    delete (Batch *)handle;
+    return TL_OK;
+}
+
+// autotuner: candidate LDS budgets and persistent grid sizes, timed on device
+// copies of a sample of the batch (restored from pristine copies before every
+// run, so every candidate permutes sorted input)
+extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
+                                       void *stream, tl_batch_config *best, float *best_ms) {
+    if (!items || !count || !best) {
+        set_error("tl_batch_autotune: bad arguments");
+        return TL_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t m = std::min(count, std::max<uint32_t>(1, sample));
+    const uint32_t step = std::max<uint32_t>(1, count / m);
+    std::vector<tl_batch_item> pick;
+    std::vector<void *> pristine, work, answers;
+    for (uint32_t i = 0; i < count && pick.size() < m; i += step) {
+        tl_batch_item it = items[i];
+        const size_t bytes = it.n * ((it.dtype == TL_U32 || it.dtype == TL_I32) ? 4 : 8);
+        void *p = nullptr, *w = nullptr, *a = nullptr;
+        TL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+        TL_HIP(hipMalloc(&w, std::max<size_t>(bytes, 8)));
+        TL_HIP(hipMalloc(&a, std::max<size_t>(it.nq * 8, 8)));
+        TL_HIP(hipMemcpy(p, it.A, bytes, hipMemcpyDeviceToDevice));
+        pristine.push_back(p);
+        work.push_back(w);
+        answers.push_back(a);
+        it.A = w;
+        it.ans = (uint64_t *)a;
+        pick.push_back(it);
+    }
+    float best_t = 1e30f;
+    tl_status result = TL_OK;
+    for (uint32_t lds : {32768u, 65536u, 98304u})
+        for (uint32_t wpc : {2u, 4u, 8u}) {
+            tl_batch_config cfg{lds, wpc, 0};
+            void *h = nullptr;
+            if (tl_batch_create(pick.data(), (uint32_t)pick.size(), &cfg, &h) != TL_OK) continue;
+            float t = 0.f, tsum = 0.f;
+            bool ok = true;
+            for (int rep = 0; rep < 3 && ok; rep++) {
+                for (size_t k = 0; k < pick.size(); k++) {
+                    const size_t bytes =
+                        pick[k].n * ((pick[k].dtype == TL_U32 || pick[k].dtype == TL_I32) ? 4 : 8);
+                    if (hipMemcpyAsync(work[k], pristine[k], bytes, hipMemcpyDeviceToDevice, s) !=
+                        hipSuccess)
+                        ok = false;
+                }
+                if (tl_batch_run(h, stream, 0, &t) != TL_OK) ok = false;
+                if (rep) tsum += t;
+            }
+            tl_batch_destroy(h);
+            if (ok && tsum / 2 < best_t) {
+                best_t = tsum / 2;
+                *best = cfg;
+            }
+        }
+    for (void *p : pristine) (void)hipFree(p);
+    for (void *p : work) (void)hipFree(p);
+    for (void *p : answers) (void)hipFree(p);
+    if (best_t >= 1e30f) {
+        set_error("tl_batch_autotune: no candidate ran");
+        result = TL_EUNSUPPORTED;
+    }
+    if (best_ms) *best_ms = best_t;
+    return result;
+}

@@ -1,0 +1,552 @@
+// tl_plan.hpp -- host planner of the in-place layout permutations, shared by
+// the single-array entry points and the uneven-batch engine.
+//
+// A plan is a list of launches (Op) over "runs" of equally sized, equally
+// spaced sub-arrays; its item lists and cycle tables live in one blob.
+#pragma once
+
+#include <math.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tl_internal.hpp"
+#include "tl_kernels.hpp"
+
+namespace tl {
+
+// ---------------------------------------------------------------------------
+// item runs: item i of a run lives at start + i*stride
+
+struct Run {
+    uint64_t start, stride, count;
+};
+
+__device__ inline uint64_t run_offset(const uint64_t *runs, uint32_t nruns, uint64_t idx) {
+    uint32_t lo = 0, hi = nruns;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (runs[4 * mid + 3] <= idx)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return runs[4 * lo] + (idx - runs[4 * lo + 3]) * runs[4 * lo + 1];
+}
+
+// ---------------------------------------------------------------------------
+// plan
+
+enum OpType { OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED };
+
+struct Op {
+    OpType type;
+    uint64_t runs_at = 0, nruns = 0, nitems = 0;   // item runs in the blob
+    uint64_t p[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // op parameters
+    MoveSpec ms{};
+};
+
+struct Plan {
+    std::vector<Op> ops;
+    std::vector<uint64_t> blob;
+    uint64_t max_stash = 0;   // elements
+    uint64_t max_moves = 0;
+
+    void put_runs(Op &op, const std::vector<Run> &runs) {
+        op.runs_at = blob.size();
+        op.nruns = runs.size();
+        uint64_t pre = 0;
+        for (const Run &r : runs) {
+            blob.push_back(r.start);
+            blob.push_back(r.stride);
+            blob.push_back(r.count);
+            blob.push_back(pre);
+            pre += r.count;
+        }
+        op.nitems = pre;
+    }
+    uint64_t put(const std::vector<uint64_t> &v) {
+        uint64_t at = blob.size();
+        blob.insert(blob.end(), v.begin(), v.end());
+        return at;
+    }
+};
+
+struct Tuning {
+    size_t lds_bytes = 65536;   // LDS budget of one in-LDS layout workgroup
+    uint64_t fmax = 256;        // largest merge radix
+    uint64_t stash_max = 1ull << 24;   // elements a rotation may stash
+};
+
+inline Tuning tuning_for(Ctx *c) {
+    Tuning t;
+    const char *e = getenv("TL_LDS_BYTES");
+    size_t want = e ? (size_t)atoll(e) : 163840;
+    t.lds_bytes = std::min(want, c->lds_max);
+    const char *f = getenv("TL_FMAX");
+    if (f) t.fmax = std::max<uint64_t>(2, atoll(f));
+    return t;
+}
+
+// cycles of the unit permutation of one dense-gather merge: F child
+// partitions, each [1 separator unit][b leaf units] -> [F separator units]
+// [F*b leaf units].  Fixed points are dropped.
+struct Cycles {
+    std::vector<uint64_t> start, units;
+};
+
+inline const Cycles &merge_cycles(uint64_t F, uint64_t b) {
+    static std::mutex mu;
+    static std::map<std::pair<uint64_t, uint64_t>, Cycles> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(F, b);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const uint64_t N = F * (b + 1);
+    std::vector<uint8_t> seen(N, 0);
+    Cycles cy;
+    for (uint64_t v0 = 0; v0 < N; v0++) {
+        if (seen[v0]) continue;
+        uint64_t v = v0, len = 0;
+        const size_t at = cy.units.size();
+        do {
+            seen[v] = 1;
+            cy.units.push_back(v);
+            len++;
+            const uint64_t j = v / (b + 1), c = v % (b + 1);
+            v = (c == 0) ? j : F + j * b + c - 1;
+        } while (v != v0);
+        if (len == 1)
+            cy.units.resize(at);
+        else
+            cy.start.push_back(at);
+    }
+    cy.start.push_back(cy.units.size());
+    return cache.emplace(key, std::move(cy)).first->second;
+}
+
+// ---------------------------------------------------------------------------
+// planners
+
+struct Planner {
+    Plan &pl;
+    Tuning tu;
+    uint32_t W;   // key width in bytes
+
+    uint64_t cap() const { return tu.lds_bytes / W; }
+
+    void lds(const std::vector<Run> &runs, uint64_t n, uint32_t kind, uint64_t b = 1,
+             uint64_t rows = 0) {
+        if (runs.empty() || n <= 1) return;
+        Op op;
+        op.type = OP_LDS;
+        pl.put_runs(op, runs);
+        if (op.nitems == 0) return;
+        uint64_t per = std::max<uint64_t>(1, cap() / n);
+        if (per > 64) per = 64;
+        op.p[0] = n;
+        op.p[1] = kind;
+        op.p[2] = b;
+        op.p[3] = rows;
+        op.p[4] = per;
+        pl.ops.push_back(op);
+    }
+
+    // rotate right by k every region [start, start+len) of `runs`
+    void rotate_right(const std::vector<Run> &runs, uint64_t len, uint64_t k) {
+        if (runs.empty() || len < 2) return;
+        k %= len;
+        if (k == 0) return;
+        uint64_t items = 0;
+        for (auto &r : runs) items += r.count;
+        const uint64_t small = std::min(k, len - k);
+        if (items == 1 && small <= tu.stash_max && len > 4 * small) {
+            const uint64_t P = runs[0].start;
+            const uint64_t CH = 256 * 16;
+            if (k <= len - k) {
+                // [X][Y] -> [Y][X] with Y small: stash Y, slide X right by k
+                copy(P + len - k, 1, k, /*to_stash=*/true);
+                chained_shift(P, len - k, k, true, CH);
+                copy(P, 1, k, /*to_stash=*/false);
+            } else {
+                // X = [P, P+len-k) small: stash X, slide Y left by len-k
+                const uint64_t x = len - k;
+                copy(P, 1, x, true);
+                chained_shift(P + x, k, x, false, CH);
+                copy(P + k, 1, x, false);
+            }
+            return;
+        }
+        // three reversals: whole, then the two parts
+        Op a;
+        a.type = OP_REVERSE;
+        pl.put_runs(a, runs);
+        a.p[0] = len;
+        pl.ops.push_back(a);
+        std::vector<Run> r1 = runs, r2 = runs;
+        for (auto &r : r2) r.start += k;
+        Op b1;
+        b1.type = OP_REVERSE;
+        pl.put_runs(b1, r1);
+        b1.p[0] = k;
+        pl.ops.push_back(b1);
+        Op b2;
+        b2.type = OP_REVERSE;
+        pl.put_runs(b2, r2);
+        b2.p[0] = len - k;
+        pl.ops.push_back(b2);
+    }
+
+    // stash (to_stash) or unstash `count` keys at P + i*stride, i < count
+    void copy(uint64_t P, uint64_t stride, uint64_t count, bool to_stash) {
+        Op op;
+        op.type = OP_COPY;
+        op.p[0] = P;
+        op.p[1] = stride;
+        op.p[2] = count;
+        op.p[3] = to_stash;
+        pl.max_stash = std::max(pl.max_stash, count);
+        pl.ops.push_back(op);
+    }
+
+    void chained_shift(uint64_t P, uint64_t L, uint64_t k, bool right, uint64_t CH) {
+        Op op;
+        op.type = OP_CHAINED;
+        op.ms.kind = right ? MV_SHIFT_RIGHT : MV_SHIFT_LEFT;
+        op.ms.P = P;
+        op.ms.L = L;
+        op.ms.k = k;
+        op.ms.CH = CH;
+        op.ms.nchunk = (L + CH - 1) / CH;
+        op.p[0] = op.ms.nchunk;
+        pl.max_moves = std::max(pl.max_moves, op.p[0]);
+        pl.ops.push_back(op);
+    }
+
+    // ---- dense gather: every item of `runs` holds M rows of b leaf keys and
+    //      one separator -> [M separators][M*b leaf keys]
+    void gather_dense(const std::vector<Run> &runs, uint64_t M, uint64_t b) {
+        if (runs.empty() || M == 0) return;
+        const uint64_t k1 = b + 1;
+        // a single row always goes to LDS, whatever the budget
+        if (M * k1 <= cap() || M == 1) {
+            lds(runs, M * k1, LDS_GATHER, b, M);
+            return;
+        }
+        const uint64_t cap_r = std::max<uint64_t>(1, cap() / k1);
+        const uint64_t tiles = (M + cap_r - 1) / cap_r;
+        const double lv = ceil(log((double)tiles) / log((double)tu.fmax) - 1e-9);
+        const uint64_t levels = std::max<uint64_t>(1, (uint64_t)lv);
+        uint64_t F = (uint64_t)ceil(pow((double)tiles, 1.0 / (double)levels) - 1e-9);
+        F = std::max<uint64_t>(2, std::min(F, M));
+        const uint64_t q = (M + F - 1) / F;
+        const uint64_t Fp = M / q;
+        const uint64_t qf = M - Fp * q;
+        // children of every parent item: a front partition of qf rows and Fp
+        // partitions of q rows.  A run of parents yields either one run per
+        // parent (children equally spaced inside it) or one run per child slot
+        // (children equally spaced across parents), whichever is shorter.
+        std::vector<Run> full, front, parents;
+        for (const Run &r : runs) {
+            if (r.count <= Fp) {
+                for (uint64_t i = 0; i < r.count; i++)
+                    full.push_back({r.start + i * r.stride + qf * k1, q * k1, Fp});
+            } else {
+                for (uint64_t j = 0; j < Fp; j++)
+                    full.push_back({r.start + (qf + j * q) * k1, r.stride, r.count});
+            }
+            front.push_back({r.start, r.stride, r.count});
+            parents.push_back({r.start + qf * k1, r.stride, r.count});
+        }
+        gather_dense(full, q, b);
+        if (qf) gather_dense(front, qf, b);
+        // merge the Fp children of every parent
+        const Cycles &cy = merge_cycles(Fp, b);
+        if (cy.start.size() > 1) {
+            Op op;
+            op.type = OP_UNIT_MERGE;
+            pl.put_runs(op, parents);
+            op.p[0] = q;                       // unit
+            op.p[1] = pl.put(cy.start);        // cycle starts
+            op.p[2] = cy.start.size() - 1;     // cycles
+            op.p[3] = pl.put(cy.units);        // cycle units
+            pl.ops.push_back(op);
+        }
+        // [S_f][L_f][S_*][L_*] -> [S_f][S_*][L_f][L_*]: rotate [L_f][S_*] left
+        if (qf) {
+            std::vector<Run> mid;
+            for (const Run &r : runs) mid.push_back({r.start + qf, r.stride, r.count});
+            const uint64_t len = qf * b + Fp * q;
+            rotate_right(mid, len, Fp * q);
+        }
+    }
+
+    // ---- sparse gather for the mvEB: every item holds r rows of (l leaf keys,
+    //      1 root) followed by untouched keys
+    void gather_sparse(const std::vector<Run> &runs, uint64_t r, uint64_t l) {
+        if (runs.empty() || r == 0) return;
+        uint64_t items = 0;
+        for (auto &x : runs) items += x.count;
+        const uint64_t part = r * (l + 1);
+        const bool roots_fit = r * W <= 65536;
+        if (roots_fit && (items >= 32 || part <= (1ull << 22))) {
+            Op op;
+            op.type = OP_SPARSE_STREAM;
+            pl.put_runs(op, runs);
+            op.p[0] = r;
+            op.p[1] = l;
+            pl.ops.push_back(op);
+            return;
+        }
+        for (const Run &x : runs)
+            for (uint64_t i = 0; i < x.count; i++) {
+                const uint64_t P = x.start + i * x.stride;
+                copy(P + l, l + 1, r, true);   // stash roots
+                Op op;
+                op.type = OP_CHAINED;
+                op.ms.kind = MV_SPARSE_SLIDE;
+                op.ms.P = P;
+                op.ms.r = r;
+                op.ms.l = l;
+                op.ms.CH = std::min<uint64_t>(l, 256 * 16);
+                op.ms.nchunk = (l + op.ms.CH - 1) / op.ms.CH;
+                op.p[0] = r * op.ms.nchunk;
+                pl.max_moves = std::max(pl.max_moves, op.p[0]);
+                pl.ops.push_back(op);
+                copy(P, 1, r, false);   // roots to the front
+            }
+    }
+};
+
+// ---- B-tree (gpu/src/btree-cycles.cuh:62-104) -----------------------------
+inline void plan_btree(Planner &P, uint64_t n, uint64_t b) {
+    if (n <= b) return;   // one node, already sorted
+    if (n <= P.cap()) {
+        P.lds({{0, 0, 1}}, n, LDS_BTREE, b);
+        return;
+    }
+    const uint64_t k1 = b + 1;
+    const uint32_t h = ilog_base(n, k1);
+    const uint64_t Pw = ipow64(k1, h);
+    if (n != Pw * k1 - 1) {
+        const uint64_t nint = Pw - 1, nleaf = n - nint;
+        const uint64_t rem = nleaf % b, l = nleaf - rem, i = l / b;
+        P.gather_dense({{0, 0, 1}}, i, b);
+        P.rotate_right({{i, 0, 1}}, nleaf + nint - i, nint - i);
+        n = nint;
+    }
+    while (n > b) {
+        if (n <= P.cap()) {
+            P.lds({{0, 0, 1}}, n, LDS_BTREE, b);
+            break;
+        }
+        P.gather_dense({{0, 0, 1}}, n / k1, b);
+        n /= k1;
+    }
+}
+
+// ---- BST (gpu/src/bst-hybrid.cuh:95-148) ----------------------------------
+inline void plan_bst(Planner &P, uint64_t n) {
+    if (n <= 1) return;
+    if (n <= P.cap()) {
+        P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
+        return;
+    }
+    uint32_t h = floor_log2_64(n);
+    uint32_t d;
+    if (n != (1ull << (h + 1)) - 1) {
+        const uint64_t nint = (1ull << h) - 1, nleaf = n - nint;
+        P.gather_dense({{0, 0, 1}}, nleaf, 1);
+        P.rotate_right({{nleaf, 0, 1}}, nint, nint - nleaf);
+        n = nint;
+        d = h;
+    } else {
+        d = h + 1;
+    }
+    const uint32_t K = (P.W == 4) ? 6 : 5;
+    if (n <= P.cap() || d < 2 * K + 2) {
+        P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
+        return;
+    }
+    Op one;
+    one.type = OP_BITREV;
+    one.p[0] = (uint64_t)(int64_t)-1;
+    one.p[1] = d;
+    one.p[2] = K;
+    P.pl.ops.push_back(one);
+    for (uint32_t t = 2 * K; t < d; t++) {
+        Op two;
+        two.type = OP_BITREV;
+        two.p[0] = (1ull << t) - 1;
+        two.p[1] = t;
+        two.p[2] = K;
+        P.pl.ops.push_back(two);
+    }
+    P.lds({{0, 0, 1}}, std::min<uint64_t>(n, (1ull << (2 * K)) - 1), LDS_LEVELREV);
+}
+
+// ---- modified vEB (gpu/src/mveb-cycles.cuh:114-440) -----------------------
+struct VebBatch {
+    uint64_t n;
+    std::vector<Run> runs;
+};
+
+inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
+                          std::vector<VebBatch> &leaves_lds, VebBatch B, uint32_t depth) {
+    if (B.n <= 1 || B.runs.empty()) return;
+    if (B.n <= P.cap()) {
+        leaves_lds.push_back(B);
+        return;
+    }
+    if (depth_ops.size() <= depth) depth_ops.resize(depth + 1);
+    // ops of this depth are appended to a private plan and moved afterwards
+    Plan sub;
+    Planner Q{sub, P.tu, P.W};
+    sub.blob.swap(P.pl.blob);
+    const uint64_t n = B.n;
+    const uint32_t d = floor_log2_64(n) + 1;
+    uint32_t rd, ld;
+    mveb_split(d, rd, ld);
+    const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
+    std::vector<VebBatch> kids;
+    if (n == (1ull << d) - 1) {
+        Q.gather_sparse(B.runs, r, l);
+        bool contiguous = (rd == ld);
+        for (auto &x : B.runs)
+            if (x.count > 1 && x.stride != n) contiguous = false;
+        if (contiguous) {
+            VebBatch c{l, {}};
+            for (auto &x : B.runs) c.runs.push_back({x.start, l, x.count * (n / l)});
+            kids.push_back(c);
+        } else {
+            VebBatch roots{r, B.runs};
+            VebBatch lv{l, {}};
+            for (auto &x : B.runs)
+                for (uint64_t i = 0; i < x.count; i++)
+                    lv.runs.push_back({x.start + i * x.stride + r, l, r + 1});
+            kids.push_back(roots);
+            kids.push_back(lv);
+        }
+    } else {
+        for (auto &x : B.runs)
+            for (uint64_t i = 0; i < x.count; i++) {
+                const uint64_t off = x.start + i * x.stride;
+                const uint64_t nf = (n - r) / l, inc = n - r - nf * l;
+                Q.gather_sparse({{off, 0, 1}}, nf, l);
+                if (nf < r) Q.rotate_right({{off + nf, 0, 1}}, n - nf, r - nf);
+                kids.push_back({r, {{off, 0, 1}}});
+                if (nf) kids.push_back({l, {{off + r, l, nf}}});
+                if (inc) kids.push_back({inc, {{off + n - inc, 0, 1}}});
+            }
+    }
+    sub.blob.swap(P.pl.blob);
+    P.pl.max_stash = std::max(P.pl.max_stash, sub.max_stash);
+    P.pl.max_moves = std::max(P.pl.max_moves, sub.max_moves);
+    for (auto &op : sub.ops) depth_ops[depth].push_back(op);
+    for (auto &k : kids) plan_mveb_rec(P, depth_ops, leaves_lds, k, depth + 1);
+}
+
+inline void plan_mveb(Planner &P, uint64_t n) {
+    if (n <= 1) return;
+    std::vector<std::vector<Op>> depth_ops;
+    std::vector<VebBatch> leaves;
+    plan_mveb_rec(P, depth_ops, leaves, VebBatch{n, {{0, 0, 1}}}, 0);
+    for (auto &ops : depth_ops)
+        for (auto &op : ops) P.pl.ops.push_back(op);
+    // group LDS leaves by size
+    std::map<uint64_t, std::vector<Run>> by_n;
+    for (auto &lb : leaves)
+        for (auto &r : lb.runs) by_n[lb.n].push_back(r);
+    for (auto &kv : by_n) P.lds(kv.second, kv.first, LDS_MVEB);
+}
+
+// ---------------------------------------------------------------------------
+// host-side bounds check of a whole plan: every launch must stay inside
+// [0, N).  A failure is an internal planning error, reported before any
+// kernel runs.
+inline uint64_t runs_end(const Plan &pl, const Op &op, uint64_t item_len) {
+    uint64_t end = 0;
+    for (uint64_t i = 0; i < op.nruns; i++) {
+        const uint64_t *r = &pl.blob[op.runs_at + 4 * i];
+        if (r[2] == 0) continue;
+        end = std::max(end, r[0] + (r[2] - 1) * r[1] + item_len);
+    }
+    return end;
+}
+
+inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
+    for (size_t k = 0; k < pl.ops.size(); k++) {
+        const Op &op = pl.ops[k];
+        uint64_t end = 0;
+        bool neg = false;
+        switch (op.type) {
+        case OP_LDS: end = runs_end(pl, op, op.p[0]); break;
+        case OP_REVERSE: end = runs_end(pl, op, op.p[0]); break;
+        case OP_BITREV: {
+            const int64_t base = (int64_t)op.p[0];
+            const uint32_t D = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
+            if (base < -1 || D < 2 * K || D > 62) neg = true;
+            end = (uint64_t)(base + (int64_t)(1ull << D));
+            break;
+        }
+        case OP_UNIT_MERGE: {
+            uint64_t mx = 0;
+            const uint64_t ncyc = op.p[2];
+            const uint64_t units = pl.blob[op.p[1] + ncyc];
+            for (uint64_t i = 0; i < units; i++) mx = std::max(mx, pl.blob[op.p[3] + i]);
+            end = runs_end(pl, op, (mx + 1) * op.p[0]);
+            break;
+        }
+        case OP_SPARSE_STREAM: end = runs_end(pl, op, op.p[0] * (op.p[1] + 1)); break;
+        case OP_COPY: end = op.p[0] + (op.p[2] ? (op.p[2] - 1) * op.p[1] + 1 : 0); break;
+        case OP_CHAINED: {
+            const MoveSpec &m = op.ms;
+            if (m.kind == MV_SPARSE_SLIDE) {
+                end = m.P + m.r * (m.l + 1);
+                if (m.CH == 0 || m.CH > 256 * 16 || m.CH > m.l) neg = true;
+            } else if (m.kind == MV_SHIFT_RIGHT) {
+                end = m.P + m.L + m.k;
+            } else {
+                end = m.P + m.L;
+                if (m.k > m.P) neg = true;
+            }
+            if (m.CH == 0 || m.CH > 256 * 16) neg = true;
+            break;
+        }
+        }
+        if (neg || end > N) {
+            why = "internal plan error: op " + std::to_string(k) + " (type " +
+                  std::to_string((int)op.type) + ") reaches " + std::to_string(end) +
+                  " of " + std::to_string(N);
+            return false;
+        }
+    }
+    return true;
+}
+
+inline bool width_of(tl_dtype dt, uint32_t &W) {
+    switch (dt) {
+    case TL_U32:
+    case TL_I32: W = 4; return true;
+    case TL_U64:
+    case TL_I64: W = 8; return true;
+    }
+    return false;
+}
+
+enum Layout { L_BST, L_BTREE, L_MVEB };
+
+inline void build_plan(Plan &pl, const Tuning &tu, Layout lay, uint64_t n, uint64_t b, uint32_t W) {
+    Planner P{pl, tu, W};
+    if (lay == L_BST)
+        plan_bst(P, n);
+    else if (lay == L_BTREE)
+        plan_btree(P, n, b);
+    else
+        plan_mveb(P, n);
+}
+
+}   // namespace tl

@@ -65,7 +65,31 @@ _SIGS = {
                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                        ctypes.POINTER(ctypes.c_uint64)],
 }
+class BatchItem(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("n", ctypes.c_uint64), ("dtype", ctypes.c_int),
+                ("layout", ctypes.c_uint32), ("b", ctypes.c_uint64), ("Q", ctypes.c_void_p),
+                ("nq", ctypes.c_uint64), ("ans", ctypes.c_void_p)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("lds_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
 _OPTIONAL = {
+    "tl_batch_create": ([ctypes.POINTER(BatchItem), ctypes.c_uint32, ctypes.POINTER(BatchConfig),
+                         ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "tl_batch_run": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                      ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "tl_batch_run_separate": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+                              ctypes.c_int),
+    "tl_batch_stats": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+                       ctypes.c_int),
+    "tl_batch_autotune": ([ctypes.POINTER(BatchItem), ctypes.c_uint32, ctypes.c_uint32,
+                           ctypes.c_void_p, ctypes.POINTER(BatchConfig),
+                           ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "tl_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "tl_learned_model_bytes": ([ctypes.c_uint32], ctypes.c_uint64),
     "tl_learned_fit": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
                         ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
@@ -268,3 +292,72 @@ def bench_learned(A, keys, Q, plain_ans, b, steps, warmup, n, num_leaves=1 << 16
     return {"ms": round(ms, 4), "Mq_s": round(nq / ms / 1e3, 2), "fit_ms": round(fit_ms, 3),
             "max_rank_err": learned_max_err(model), "fallback_rate": fallbacks / nq,
             "answers_equal_plain": same, "num_leaves": num_leaves}
+
+
+# ---- uneven batch --------------------------------------------------------------
+
+class Batch:
+    """Owns a tl_batch handle over caller tensors (kept alive by this object).
+    arrays: list of dicts with A (tensor), kind, layout ('bst'|'btree'|'mveb'),
+    b, Q (tensor or None), ans (int64 tensor or None)."""
+
+    def __init__(self, arrays, lds_bytes=0, wg_per_cu=0):
+        self.arrays = arrays
+        items = (BatchItem * len(arrays))()
+        for i, a in enumerate(arrays):
+            items[i].A = a["A"].data_ptr()
+            items[i].n = a["A"].numel()
+            items[i].dtype = DTYPE[a["kind"]]
+            items[i].layout = LAYOUT[a["layout"]]
+            items[i].b = a.get("b", 1)
+            q = a.get("Q")
+            items[i].Q = q.data_ptr() if q is not None else None
+            items[i].nq = q.numel() if q is not None else 0
+            items[i].ans = a["ans"].data_ptr() if q is not None else None
+        self._items = items
+        cfg = BatchConfig(lds_bytes, wg_per_cu, 0)
+        h = ctypes.c_void_p()
+        _check(lib().tl_batch_create(items, len(arrays), ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def stats(self):
+        o, t, s = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().tl_batch_stats(self.h, ctypes.byref(o), ctypes.byref(t), ctypes.byref(s)))
+        return {"ops": o.value, "tasks": t.value, "smem": s.value}
+
+    def run(self, graph=True, stream=None, timed=True):
+        return _timed_call(lib().tl_batch_run, self.h, _stream(stream), 1 if graph else 0,
+                           timed=timed)
+
+    def run_separate(self, stream=None, timed=True):
+        return _timed_call(lib().tl_batch_run_separate, self.h, _stream(stream), timed=timed)
+
+    def close(self):
+        if self.h:
+            lib().tl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_autotune(arrays, sample=256, stream=None):
+    items = (BatchItem * len(arrays))()
+    for i, a in enumerate(arrays):
+        items[i].A = a["A"].data_ptr()
+        items[i].n = a["A"].numel()
+        items[i].dtype = DTYPE[a["kind"]]
+        items[i].layout = LAYOUT[a["layout"]]
+        items[i].b = a.get("b", 1)
+        q = a.get("Q")
+        items[i].Q = q.data_ptr() if q is not None else None
+        items[i].nq = q.numel() if q is not None else 0
+        items[i].ans = a["ans"].data_ptr() if q is not None else None
+    best = BatchConfig()
+    ms = ctypes.c_float()
+    _check(lib().tl_batch_autotune(items, len(arrays), sample, _stream(stream), ctypes.byref(best),
+                                   ctypes.byref(ms)))
+    return {"lds_bytes": best.lds_bytes, "wg_per_cu": best.wg_per_cu, "sample_ms": ms.value}
