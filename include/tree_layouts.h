@@ -139,11 +139,14 @@ typedef struct {
 } tl_batch_item;
 
 /* lds_bytes: LDS budget of the in-LDS sub-layouts (0 = 64 KiB);
- * wg_per_cu: persistent workgroups per compute unit (0 = 4). */
+ * wg_per_cu: persistent workgroups per compute unit (0 = 4);
+ * wait_ms: bound on any single in-kernel wait of the persistent launch
+ * (0 = the library default, 4000 ms or $TL_WAIT_MS); a wait that reaches it
+ * makes the run report TL_ETIMEOUT. */
 typedef struct {
     uint32_t lds_bytes;
     uint32_t wg_per_cu;
-    uint32_t reserved;
+    uint32_t wait_ms;
 } tl_batch_config;
 
 /* Plans every array (the same plans as the single-array permutations) and
@@ -173,6 +176,20 @@ tl_status tl_batch_destroy(void *handle);
  * on the host; it has no counterpart in the reference. */
 tl_status tl_plan_export(uint32_t layout, uint64_t n, uint64_t b, tl_dtype dtype,
                          uint64_t lds_bytes, uint64_t *buf, uint64_t cap, uint64_t *used);
+
+/* Dry run of tl_batch_create for a device with num_cus compute units and
+ * lds_max LDS bytes per workgroup (0 = 256 and 160 KiB): builds the batch,
+ * checks that every op's predecessor ends before the op's first sub-task,
+ * and copies out the op records (ops_cap bytes), the sub-task list
+ * (tasks_cap bytes) and the blob (blob_cap words), each only when its buffer
+ * is large enough.  sizes[0..4] receive the op count, the sub-task count, the
+ * blob words, the LDS bytes per workgroup and the size of one op record.
+ * Lets a test replay the persistent launch on the host; no counterpart in
+ * the reference. */
+tl_status tl_batch_export(const tl_batch_item *items, uint32_t count, const tl_batch_config *cfg,
+                          uint32_t num_cus, uint32_t lds_max, void *ops_out, uint64_t ops_cap,
+                          void *tasks_out, uint64_t tasks_cap, uint64_t *blob_out,
+                          uint64_t blob_cap, uint64_t *sizes);
 
 #ifdef __cplusplus
 }

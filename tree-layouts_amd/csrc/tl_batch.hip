@@ -239,7 +239,8 @@ __device__ void ex_copy(const BOp &op, uint64_t sub, T *stash) {
 constexpr int MOVE_PER = 16;
 
 template <typename T>
-__device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32_t *err) {
+__device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32_t *err,
+                           uint64_t limit) {
     T *A = (T *)op.A;
     uint32_t *flags = flagbase + op.flags_at;
     uint64_t src, dst, len;
@@ -250,10 +251,12 @@ __device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32
         const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
         if (idx < len) v[j] = A[src + idx];
     }
+    // every wave's loads complete before the chunk is published as read
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        wait_deps(op.ms, t, dst, len, flags, err);
+        __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_deps(op.ms, t, dst, len, flags, err, limit);
     }
     __syncthreads();
 #pragma unroll
@@ -402,7 +405,8 @@ __device__ void ex_query(const BOp &op, uint64_t sub, const uint64_t *blob, unsi
 
 template <typename T>
 __device__ void ex_permute_op(const BOp &op, uint64_t sub, const uint64_t *blob,
-                              unsigned char *smem, T *stash, uint32_t *flags, uint32_t *err) {
+                              unsigned char *smem, T *stash, uint32_t *flags, uint32_t *err,
+                              uint64_t limit) {
     switch (op.type) {
     case OP_LDS: ex_lds<T>(op, sub, blob, smem); break;
     case OP_REVERSE: ex_reverse<T>(op, sub, blob); break;
@@ -415,25 +419,19 @@ __device__ void ex_permute_op(const BOp &op, uint64_t sub, const uint64_t *blob,
     case OP_UNIT_MERGE: ex_unit_merge<T>(op, sub, blob); break;
     case OP_SPARSE_STREAM: ex_sparse_stream<T>(op, sub, blob, smem); break;
     case OP_COPY: ex_copy<T>(op, sub, stash); break;
-    case OP_CHAINED: ex_chained<T>(op, sub, flags, err); break;
+    case OP_CHAINED: ex_chained<T>(op, sub, flags, err, limit); break;
     default: break;
     }
 }
 
 // bounded wait for an op's completion counter
-__device__ inline bool wait_done(unsigned long long *done, uint64_t want, uint32_t *err) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1u << 24)) {
-            atomicOr(err, 2u);
-            return false;
-        }
-        if ((spins & 255u) == 0u &&
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
-            return false;
-    }
-    return true;
+__device__ inline bool wait_done(unsigned long long *done, uint64_t want, uint32_t *err,
+                                 uint64_t limit) {
+    return bounded_wait(
+        [done, want]() {
+            return __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want;
+        },
+        err, 2u, limit);
 }
 
 __global__ __launch_bounds__(BATCH_THREADS) void k_batch(const BOp *__restrict__ ops,
@@ -442,7 +440,7 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_batch(const BOp *__restrict__
                                                           unsigned long long *done,
                                                           unsigned long long *ticket,
                                                           uint32_t *flags, void *stash,
-                                                          uint32_t *err) {
+                                                          uint32_t *err, uint64_t limit) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint64_t s_t;
     for (;;) {
@@ -453,7 +451,8 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_batch(const BOp *__restrict__
         if (t >= ntasks) return;
         const BTask tk = tasks[t];
         const BOp &op = ops[tk.op];
-        if (threadIdx.x == 0 && op.prev >= 0) wait_done(&done[op.prev], ops[op.prev].ntasks, err);
+        if (threadIdx.x == 0 && op.prev >= 0)
+            wait_done(&done[op.prev], ops[op.prev].ntasks, err, limit);
         __syncthreads();
         if (op.type == OP_QUERY) {
             switch (op.dtype) {
@@ -463,13 +462,15 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_batch(const BOp *__restrict__
             default: ex_query<int64_t>(op, tk.sub, blob, smem); break;
             }
         } else if (op.width == 4) {
-            ex_permute_op<uint32_t>(op, tk.sub, blob, smem, (uint32_t *)stash, flags, err);
+            ex_permute_op<uint32_t>(op, tk.sub, blob, smem, (uint32_t *)stash, flags, err, limit);
         } else {
-            ex_permute_op<uint64_t>(op, tk.sub, blob, smem, (uint64_t *)stash, flags, err);
+            ex_permute_op<uint64_t>(op, tk.sub, blob, smem, (uint64_t *)stash, flags, err, limit);
         }
+        // every wave's stores reach agent scope before the sub-task is counted
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (threadIdx.x == 0)
-            __hip_atomic_fetch_add(&done[tk.op], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&done[tk.op], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -496,7 +497,7 @@ struct Batch {
     std::vector<BOp> ops;
     std::vector<BTask> tasks;
     std::vector<uint64_t> blob;
-    uint64_t nflags = 0, stash_bytes = 0;
+    uint64_t nflags = 0, stash_bytes = 0, wait_ticks = 0;
     // device
     BOp *d_ops = nullptr;
     BTask *d_tasks = nullptr;
@@ -583,7 +584,7 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     B.nflags = 0;
     uint64_t stash_elems = 0;
     Tuning tu;
-    tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max);
+    tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
     struct Pending {
         uint32_t op, depth, array;
     };
@@ -686,12 +687,13 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     for (const Pending &pd : order)
         for (uint64_t s = 0; s < B.ops[pd.op].ntasks; s++) B.tasks.push_back({pd.op, 0, s});
     B.smem = std::max<uint64_t>(max_smem, 1024);
-    if (B.smem > c->lds_max) {
+    if (B.smem + LDS_STATIC_RESERVE > c->lds_max) {
         set_error("tl_batch: LDS budget exceeds the device limit");
         return TL_EINVAL;
     }
     B.stash_bytes = std::max<uint64_t>(stash_elems * 8, 64);
     B.grid = (uint32_t)std::max<uint64_t>(1, (uint64_t)c->num_cus * B.cfg.wg_per_cu);
+    B.wait_ticks = B.cfg.wait_ms ? (uint64_t)B.cfg.wait_ms * c->wall_khz : c->wait_ticks;
     return TL_OK;
 }
 
@@ -702,7 +704,10 @@ static tl_status upload_batch(Batch &B) {
             *dst = nullptr;
         }
         TL_HIP(hipMalloc(dst, std::max<size_t>(bytes, 64)));
-        if (bytes) TL_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        if (bytes && src)
+            TL_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        else
+            TL_HIP(hipMemset(*dst, 0, std::max<size_t>(bytes, 64)));
         return TL_OK;
     };
     TL_TRY(up((void **)&B.d_ops, B.ops.data(), B.ops.size() * sizeof(BOp)));
@@ -721,7 +726,7 @@ static void launch_batch(Batch &B, hipStream_t s) {
                        B.d_flags, B.nflags, B.d_ticket, B.d_err);
     hipLaunchKernelGGL(k_batch, dim3(B.grid), dim3(BATCH_THREADS), B.smem, s, B.d_ops, B.d_tasks,
                        (uint64_t)B.tasks.size(), B.d_blob, B.d_done, B.d_ticket, B.d_flags,
-                       B.d_stash, B.d_err);
+                       B.d_stash, B.d_err, B.wait_ticks);
 }
 
 }   // namespace tl
@@ -914,4 +919,59 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
     }
     if (best_ms) *best_ms = best_t;
     return result;
+}
+
+// host-only dry run of tl_batch_create (no device): builds the batch for a
+// device with `num_cus` compute units and `lds_max` LDS bytes, checks that
+// every op's predecessor ends before the op's first sub-task, and copies the
+// op table, the sub-task list and the blob out (each only when its buffer is
+// large enough).  sizes[0..4] = ops, sub-tasks, blob words, LDS bytes,
+// sizeof(op record).
+extern "C" tl_status tl_batch_export(const tl_batch_item *items, uint32_t count,
+                                     const tl_batch_config *cfg, uint32_t num_cus,
+                                     uint32_t lds_max, void *ops_out, uint64_t ops_cap,
+                                     void *tasks_out, uint64_t tasks_cap, uint64_t *blob_out,
+                                     uint64_t blob_cap, uint64_t *sizes) {
+    if ((count && !items) || !sizes) {
+        set_error("tl_batch_export: bad arguments");
+        return TL_EINVAL;
+    }
+    Ctx c;
+    c.num_cus = num_cus ? num_cus : 256;
+    c.lds_max = lds_max ? lds_max : 163840;
+    Batch B;
+    B.items.assign(items, items + count);
+    B.cfg = cfg ? *cfg : tl_batch_config{65536, 4, 0};
+    if (B.cfg.lds_bytes == 0) B.cfg.lds_bytes = 65536;
+    if (B.cfg.wg_per_cu == 0) B.cfg.wg_per_cu = 4;
+    TL_TRY(build_batch(B, &c));
+    std::vector<int64_t> first(B.ops.size(), -1), last(B.ops.size(), -1);
+    for (size_t t = 0; t < B.tasks.size(); t++) {
+        const uint32_t o = B.tasks[t].op;
+        if (first[o] < 0) first[o] = (int64_t)t;
+        last[o] = (int64_t)t;
+    }
+    for (size_t o = 0; o < B.ops.size(); o++) {
+        const int64_t p = B.ops[o].prev;
+        if (p >= 0 && !(last[p] < first[o])) {
+            set_error("tl_batch_export: op " + std::to_string(o) + " precedes its predecessor");
+            return TL_EINVAL;
+        }
+        if ((uint64_t)(last[o] - first[o] + 1) != B.ops[o].ntasks) {
+            set_error("tl_batch_export: sub-tasks of op " + std::to_string(o) + " not contiguous");
+            return TL_EINVAL;
+        }
+    }
+    sizes[0] = B.ops.size();
+    sizes[1] = B.tasks.size();
+    sizes[2] = B.blob.size();
+    sizes[3] = B.smem;
+    sizes[4] = sizeof(BOp);
+    if (ops_out && ops_cap >= B.ops.size() * sizeof(BOp))
+        memcpy(ops_out, B.ops.data(), B.ops.size() * sizeof(BOp));
+    if (tasks_out && tasks_cap >= B.tasks.size() * sizeof(BTask))
+        memcpy(tasks_out, B.tasks.data(), B.tasks.size() * sizeof(BTask));
+    if (blob_out && blob_cap >= B.blob.size())
+        memcpy(blob_out, B.blob.data(), B.blob.size() * 8);
+    return TL_OK;
 }

@@ -54,6 +54,8 @@ struct Ctx {
     void *scratch[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t scratch_bytes[4] = {0, 0, 0, 0};
     uint32_t *err_flag = nullptr;   // device word set by bounded waits on timeout
+    uint64_t wall_khz = 100000;     // rate of the device wall clock
+    uint64_t wait_ticks = 0;        // bound on one in-kernel wait, wall-clock ticks
 };
 
 tl_status get_ctx(Ctx **out);

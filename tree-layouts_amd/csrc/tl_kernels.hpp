@@ -104,26 +104,46 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
     }
 }
 
-// bounded wait: gives up after ~2^24 polls, or at once when another
-// workgroup already gave up (then the result is reported as a timeout)
-__device__ inline bool wait_flag(uint32_t *f, uint32_t *err) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (1u << 24)) {
-            atomicOr(err, 1u);
-            return false;
+// Bounded spin wait.  Polls with relaxed agent-scope loads (no cache
+// maintenance per poll), backs off with s_sleep, and issues one agent-scope
+// acquire fence once `ready()` holds.  Gives up after `limit` ticks of the
+// constant-rate wall clock, or soon after another workgroup gave up
+// (err != 0); a give-up ORs `code` into err and is reported as a timeout.
+template <typename Ready>
+__device__ inline bool bounded_wait(Ready ready, uint32_t *err, uint32_t code, uint64_t limit) {
+    if (!ready()) {
+        const uint64_t t0 = (uint64_t)wall_clock64();
+        for (uint32_t spins = 1;; spins++) {
+            if (spins < 16)
+                __builtin_amdgcn_s_sleep(1);
+            else if (spins < 256)
+                __builtin_amdgcn_s_sleep(8);
+            else
+                __builtin_amdgcn_s_sleep(32);
+            if (ready()) break;
+            if ((spins & 7u) == 0u) {
+                if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                    return false;
+                if ((uint64_t)wall_clock64() - t0 > limit) {
+                    atomicOr(err, code);
+                    return false;
+                }
+            }
         }
-        if ((spins & 255u) == 0u &&
-            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
-            return false;
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
+}
+
+__device__ inline bool wait_flag(uint32_t *f, uint32_t *err, uint64_t limit) {
+    return bounded_wait(
+        [f]() { return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u; },
+        err, 1u, limit);
 }
 
 // waits for every move t' < t whose source overlaps [dst, dst+len)
 __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
-                                 uint32_t *flags, uint32_t *err) {
+                                 uint32_t *flags, uint32_t *err, uint64_t limit) {
     if (m.kind == MV_SHIFT_RIGHT || m.kind == MV_SHIFT_LEFT) {
         if (dst + len <= m.P || dst >= m.P + m.L) return;
         const uint64_t lo = dst > m.P ? (dst - m.P) / m.CH : 0;
@@ -131,7 +151,7 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         if (hi >= m.nchunk) hi = m.nchunk - 1;
         for (uint64_t c = lo; c <= hi; c++) {
             const uint64_t tt = (m.kind == MV_SHIFT_RIGHT) ? m.nchunk - 1 - c : c;
-            if (tt < t && !wait_flag(&flags[tt], err)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
         }
         return;
     }
@@ -148,7 +168,7 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         const uint64_t c_lo = ov_lo / m.CH, c_hi = (ov_end - bs - 1) / m.CH;
         for (uint64_t c = c_lo; c <= c_hi; c++) {
             const uint64_t tt = (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c);
-            if (tt < t && !wait_flag(&flags[tt], err)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
         }
     }
 }
@@ -156,7 +176,7 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
 template <typename T, int PER>
 __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
                                                  uint32_t *flags, unsigned long long *counter,
-                                                 uint32_t *err) {
+                                                 uint32_t *err, uint64_t limit) {
     __shared__ uint64_t s_t;
     for (;;) {
         if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
@@ -172,10 +192,12 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
             const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
             if (idx < len) v[j] = A[src + idx];
         }
+        // every wave's loads complete before the chunk is published as read
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (threadIdx.x == 0) {
-            __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            wait_deps(ms, t, dst, len, flags, err);
+            __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wait_deps(ms, t, dst, len, flags, err, limit);
         }
         __syncthreads();
 #pragma unroll

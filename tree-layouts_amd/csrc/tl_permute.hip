@@ -267,7 +267,7 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
             unsigned long long *counter = (unsigned long long *)(flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
             hipLaunchKernelGGL((k_chained<T, 16>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
-                               flags, counter, c->err_flag);
+                               flags, counter, c->err_flag, c->wait_ticks);
             TL_LAUNCH_CHECK();
             break;
         }

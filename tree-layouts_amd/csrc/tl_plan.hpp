@@ -75,6 +75,9 @@ struct Plan {
     }
 };
 
+// LDS kept free for kernels' static shared variables next to the dynamic area
+constexpr size_t LDS_STATIC_RESERVE = 2048;
+
 struct Tuning {
     size_t lds_bytes = 65536;   // LDS budget of one in-LDS layout workgroup
     uint64_t fmax = 256;        // largest merge radix
@@ -85,7 +88,8 @@ inline Tuning tuning_for(Ctx *c) {
     Tuning t;
     const char *e = getenv("TL_LDS_BYTES");
     size_t want = e ? (size_t)atoll(e) : 163840;
-    t.lds_bytes = std::min(want, c->lds_max);
+    // the in-LDS kernel keeps 512 bytes of static LDS beside the dynamic area
+    t.lds_bytes = std::min(want, c->lds_max - LDS_STATIC_RESERVE);
     const char *f = getenv("TL_FMAX");
     if (f) t.fmax = std::max<uint64_t>(2, atoll(f));
     return t;

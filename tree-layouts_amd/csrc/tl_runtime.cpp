@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "tl_internal.hpp"
@@ -30,6 +31,12 @@ tl_status get_ctx(Ctx **out) {
         TL_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         c.num_cus = cus > 0 ? cus : 256;
         c.lds_max = lds > 0 ? (size_t)lds : 65536;
+        int khz = 0;
+        TL_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        c.wall_khz = khz > 0 ? (uint64_t)khz : 100000;
+        const char *w = getenv("TL_WAIT_MS");
+        const uint64_t wait_ms = w ? std::max<uint64_t>(1, strtoull(w, nullptr, 10)) : 4000;
+        c.wait_ticks = wait_ms * c.wall_khz;
         void *p = nullptr;
         TL_HIP(hipMalloc(&p, 256));
         TL_HIP(hipMemset(p, 0, 256));

@@ -73,7 +73,7 @@ class BatchItem(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("lds_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("wait_ms", ctypes.c_uint32)]
 
 
 _OPTIONAL = {
@@ -94,6 +94,10 @@ _OPTIONAL = {
     "tl_learned_fit": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
                         ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
                        ctypes.c_int),
+    "tl_batch_export": ([ctypes.POINTER(BatchItem), ctypes.c_uint32, ctypes.POINTER(BatchConfig),
+                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                         ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+                         ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "tl_query_btree_learned": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -301,7 +305,7 @@ class Batch:
     arrays: list of dicts with A (tensor), kind, layout ('bst'|'btree'|'mveb'),
     b, Q (tensor or None), ans (int64 tensor or None)."""
 
-    def __init__(self, arrays, lds_bytes=0, wg_per_cu=0):
+    def __init__(self, arrays, lds_bytes=0, wg_per_cu=0, wait_ms=0):
         self.arrays = arrays
         items = (BatchItem * len(arrays))()
         for i, a in enumerate(arrays):
@@ -315,7 +319,7 @@ class Batch:
             items[i].nq = q.numel() if q is not None else 0
             items[i].ans = a["ans"].data_ptr() if q is not None else None
         self._items = items
-        cfg = BatchConfig(lds_bytes, wg_per_cu, 0)
+        cfg = BatchConfig(lds_bytes, wg_per_cu, wait_ms)
         h = ctypes.c_void_p()
         _check(lib().tl_batch_create(items, len(arrays), ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
