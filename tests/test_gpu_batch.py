@@ -1,7 +1,8 @@
 """Uneven batch: every array of a mixed batch (sizes spread over four orders
 of magnitude, int32/uint64 keys, BST / B-tree b=8 / mvEB) comes out of the
-single persistent launch with the oracle's layout and the oracle's answers;
-the graph replay, the plain launch and the per-array path agree."""
+batch engine with the oracle's layout and the oracle's answers.  The per-array
+path runs; the single persistent launch is disabled (its tests are skipped
+with the reason) and must fail loudly."""
 import numpy as np
 import pytest
 import torch
@@ -55,6 +56,12 @@ def restore(arrays, host, dev):
         a["ans"].fill_(-1)
 
 
+PERSISTENT_OFF = ("the single persistent launch stalled on gfx950 (twice, the second time with "
+                  "wall-clock-bounded waits) and is disabled until its cause is shown; the "
+                  "sub-task list is checked by the host replay in test_batch_plan.py")
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
 @pytest.mark.parametrize("lds", [0, 32768])
 def test_batch_matches_oracle(lds, dev):
     arrays, host = make_batch(240, 0, 18, 5, dev)
@@ -83,6 +90,16 @@ def test_batch_separate_path_matches(dev):
     B.close()
 
 
+def test_persistent_launch_fails_loudly(dev):
+    arrays, host = make_batch(6, 2, 8, 8, dev)
+    B = tl_amd.Batch(arrays)
+    with pytest.raises(tl_amd.TLError) as e:
+        B.run(graph=False)
+    assert e.value.status == tl_amd.TL_EUNSUPPORTED
+    B.close()
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
 def test_batch_autotune_returns_a_config(dev):
     arrays, host = make_batch(90, 6, 16, 7, dev)
     cfg = tl_amd.batch_autotune(arrays, sample=30)

@@ -762,12 +762,26 @@ extern "C" tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks
     return TL_OK;
 }
 
+// The single persistent launch (k_batch) stalled on gfx950 for the 240-array
+// mixed batch of the GPU tests, first with acquire polling bounded by a poll
+// count and again with relaxed polling bounded by wall-clock time; the host
+// replay of the same sub-task list is correct and every wait targets a
+// smaller ticket, so the stall is not a scheduling deadlock of the task list.
+// Until its cause is shown the launch stays off; the sub-task list is still
+// built, exported and replayed on the host.
+static bool persistent_launch_enabled() { return false; }
+
 extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms) {
     if (!handle) {
         set_error("tl_batch_run: null handle");
         return TL_EINVAL;
     }
     Batch *B = (Batch *)handle;
+    if (!persistent_launch_enabled()) {
+        set_error("tl_batch_run: the persistent launch is disabled: it stalled on gfx950 and the "
+                  "cause is not identified; tl_batch_run_separate runs the same work");
+        return TL_EUNSUPPORTED;
+    }
     hipStream_t s = (hipStream_t)stream;
     Timer tm;
     if (use_graph) {
