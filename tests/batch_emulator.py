@@ -90,6 +90,10 @@ def move_at(m, t):
     if m.kind == pe.MV_SHIFT_LEFT:
         src = m.P + t * m.CH
         return src, src - m.k, min(m.CH, m.L - t * m.CH)
+    if m.kind == pe.MV_ROW_SLIDE:
+        j0 = (m.nchunk - 1 - t) * m.CH
+        j1 = min(m.r, j0 + m.CH)
+        return m.P + j0 * (m.l + 1), m.P + m.r + j0 * m.l, (j1 - j0) * (m.l + 1)
     i = m.r - 1 - t // m.nchunk
     c = m.nchunk - 1 - t % m.nchunk
     src = m.P + i * (m.l + 1) + c * m.CH
@@ -106,6 +110,14 @@ def awaited_moves(m, t, dst, ln):
         hi = min((dst + ln - 1 - m.P) // m.CH, m.nchunk - 1)
         for c in range(lo, hi + 1):
             tt = m.nchunk - 1 - c if m.kind == pe.MV_SHIFT_RIGHT else c
+            if tt < t:
+                out.append(tt)
+        return out
+    if m.kind == pe.MV_ROW_SLIDE:
+        dlen = ln // (m.l + 1) * m.l
+        G = m.CH * (m.l + 1)
+        for g in range((dst - m.P) // G, min((dst + dlen - 1 - m.P) // G, m.nchunk - 1) + 1):
+            tt = m.nchunk - 1 - g
             if tt < t:
                 out.append(tt)
         return out
@@ -247,7 +259,16 @@ class Replay:
                 assert self.flags.get((oid, tt)), f"move {sub} waits on unpublished {tt}"
             v = A[src:src + ln].copy()
             self.flags[(oid, sub)] = True
-            A[dst:dst + ln] = v
+            if m.kind == pe.MV_ROW_SLIDE:
+                L1 = m.l + 1
+                rows = v.reshape(-1, L1)
+                st = self.stash.setdefault(op.stash_at, {})
+                j0 = (src - m.P) // L1
+                for k in range(rows.shape[0]):
+                    st[j0 + k] = rows[k, m.l]
+                A[dst:dst + rows.shape[0] * m.l] = rows[:, :m.l].reshape(-1)
+            else:
+                A[dst:dst + ln] = v
             work = ln
         else:
             raise ValueError(t)

@@ -11,7 +11,7 @@ import oracle_lib as ol
 
 OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED = range(7)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV = range(4)
-MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE = range(3)
+MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE = range(4)
 
 
 def decode(buf):
@@ -120,6 +120,11 @@ def replay(buf, A):
                 A[P + k: P + k + L] = A[P:P + L].copy()
             elif kind == MV_SHIFT_LEFT:
                 A[P - k: P - k + L] = A[P:P + L].copy()
+            elif kind == MV_ROW_SLIDE:
+                # rows of (l keys, 1 separator): separators to the stash
+                reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
+                stash = reg[:, l].copy()
+                A[P + r: P + r * (l + 1)] = reg[:, :l].reshape(-1)
             else:
                 reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
                 A[P + r: P + r * (l + 1)] = reg[:, :l].reshape(-1)

@@ -1,8 +1,9 @@
 """GPU parity of the in-place permutations against the oracle.
 
 Small and medium sizes are compared slot by slot with the oracle's restated
-reference algorithm; every size class of the plan (in-LDS, multi-level
-gathers with front partitions, bit-reversal tiles, chained moves) is hit.
+reference algorithm; every size class of the plan (in-LDS, row slides,
+multi-level gathers with front partitions, bit-reversal tiles, chained moves)
+is hit.
 """
 import numpy as np
 import pytest
@@ -48,6 +49,17 @@ def test_permute_medium(layout, b, kind, dev):
     for n in MEDIUM:
         got, exp = _run(layout, b, kind, n, dev)
         assert np.array_equal(got, exp), (layout, b, kind, n)
+
+
+@pytest.mark.parametrize("layout,b", [("btree", 16), ("btree", 8), ("btree", 2), ("bst", 1)])
+@pytest.mark.parametrize("row_stash", ["0", "1000"])
+def test_permute_stash_budgets(layout, b, row_stash, dev, monkeypatch):
+    """Row slides off (hierarchical LDS + unit-merge gathers only) and a tiny
+    row-slide budget (both kinds of gather in one plan)."""
+    monkeypatch.setenv("TL_ROW_STASH_MAX", row_stash)
+    for n in MEDIUM + [17 ** 5 - 1, 3_000_017]:
+        got, exp = _run(layout, b, "u64", n, dev)
+        assert np.array_equal(got, exp), (layout, b, row_stash, n)
 
 
 @pytest.mark.parametrize("layout,b,n", [("bst", 1, (1 << 24) - 1), ("bst", 1, (1 << 24) + 12345),

@@ -34,7 +34,13 @@ def test_invalid_arguments_fail_without_device():
 @pytest.mark.parametrize("layout,bs", [("btree", (16, 8, 2, 3, 32, 1)), ("bst", (1,)),
                                        ("mveb", (0,))])
 @pytest.mark.parametrize("lds", [256, 2048])
-def test_plans_replay_to_oracle(layout, bs, lds):
+@pytest.mark.parametrize("row_stash", [None, "0", "40"])
+def test_plans_replay_to_oracle(layout, bs, lds, row_stash, monkeypatch):
+    """row_stash: the library's default stash budget for row slides, none (the
+    hierarchical LDS + unit-merge gathers), or 40 separators (both kinds in
+    one plan)"""
+    if row_stash is not None:
+        monkeypatch.setenv("TL_ROW_STASH_MAX", row_stash)
     sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 5000, 17 ** 3 - 1, 17 ** 3,
              33333, 65535, 65536]
     for b in bs:

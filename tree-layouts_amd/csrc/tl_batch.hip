@@ -240,7 +240,7 @@ constexpr int MOVE_PER = 16;
 
 template <typename T>
 __device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32_t *err,
-                           uint64_t limit) {
+                           uint64_t limit, T *stash) {
     T *A = (T *)op.A;
     uint32_t *flags = flagbase + op.flags_at;
     uint64_t src, dst, len;
@@ -252,18 +252,14 @@ __device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32
         if (idx < len) v[j] = A[src + idx];
     }
     // every wave's loads complete before the chunk is published as read
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    loads_done();
     __syncthreads();
     if (threadIdx.x == 0) {
         __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wait_deps(op.ms, t, dst, len, flags, err, limit);
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < MOVE_PER; j++) {
-        const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
-        if (idx < len) A[dst + idx] = v[j];
-    }
+    store_move<T, MOVE_PER>(A, stash + op.stash_at, op.ms, src, dst, len, v);
 }
 
 // ---- queries: QCHUNK queries per sub-task --------------------------------
@@ -419,7 +415,7 @@ __device__ void ex_permute_op(const BOp &op, uint64_t sub, const uint64_t *blob,
     case OP_UNIT_MERGE: ex_unit_merge<T>(op, sub, blob); break;
     case OP_SPARSE_STREAM: ex_sparse_stream<T>(op, sub, blob, smem); break;
     case OP_COPY: ex_copy<T>(op, sub, stash); break;
-    case OP_CHAINED: ex_chained<T>(op, sub, flags, err, limit); break;
+    case OP_CHAINED: ex_chained<T>(op, sub, flags, err, limit, stash); break;
     default: break;
     }
 }

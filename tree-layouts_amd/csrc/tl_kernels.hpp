@@ -72,16 +72,19 @@ __global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t
 //     to a workgroup that already started and publishes before it waits, so
 //     the scheme cannot deadlock; the wait is still bounded and reports a
 //     timeout through err instead of hanging.
-enum : uint32_t { MV_SHIFT_RIGHT = 0, MV_SHIFT_LEFT = 1, MV_SPARSE_SLIDE = 2 };
+//   MV_ROW_SLIDE: r rows of (l keys, 1 separator) at P become
+//     [r separators (left in the stash)][r*l keys]; a move carries CH whole
+//     rows, writes their keys contiguously and their separators to the stash.
+enum : uint32_t { MV_SHIFT_RIGHT = 0, MV_SHIFT_LEFT = 1, MV_SPARSE_SLIDE = 2, MV_ROW_SLIDE = 3 };
 
 struct MoveSpec {
     uint32_t kind;
     uint64_t P;       // region start
     uint64_t L;       // shift: region length
     uint64_t k;       // shift distance
-    uint64_t r, l;    // sparse slide: rows and leaf-block length
-    uint64_t CH;      // chunk length
-    uint64_t nchunk;  // chunks per region (shift) or per leaf block (slide)
+    uint64_t r, l;    // slides: rows and leaf-block length
+    uint64_t CH;      // chunk length (rows per move for the row slide)
+    uint64_t nchunk;  // chunks per region / per leaf block, or row-slide moves
 };
 
 __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uint64_t &dst,
@@ -95,6 +98,12 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
         src = m.P + t * m.CH;
         len = min(m.CH, m.L - t * m.CH);
         dst = src - m.k;
+    } else if (m.kind == MV_ROW_SLIDE) {
+        const uint64_t j0 = (m.nchunk - 1 - t) * m.CH;
+        const uint64_t j1 = min(m.r, j0 + m.CH);
+        src = m.P + j0 * (m.l + 1);
+        len = (j1 - j0) * (m.l + 1);
+        dst = m.P + m.r + j0 * m.l;
     } else {
         const uint64_t i = m.r - 1 - t / m.nchunk;
         const uint64_t c = m.nchunk - 1 - t % m.nchunk;
@@ -109,7 +118,7 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
 // acquire fence once `ready()` holds.  Gives up after `limit` ticks of the
 // constant-rate wall clock, or soon after another workgroup gave up
 // (err != 0); a give-up ORs `code` into err and is reported as a timeout.
-template <typename Ready>
+template <bool Acquire = true, typename Ready>
 __device__ inline bool bounded_wait(Ready ready, uint32_t *err, uint32_t code, uint64_t limit) {
     if (!ready()) {
         const uint64_t t0 = (uint64_t)wall_clock64();
@@ -131,15 +140,21 @@ __device__ inline bool bounded_wait(Ready ready, uint32_t *err, uint32_t code, u
             }
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (Acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
 }
 
+// chained-move flag: "the chunk was read".  The waiter only overwrites that
+// chunk and never reads data the flag publishes, so no acquire is needed.
 __device__ inline bool wait_flag(uint32_t *f, uint32_t *err, uint64_t limit) {
-    return bounded_wait(
+    return bounded_wait<false>(
         [f]() { return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u; },
         err, 1u, limit);
 }
+
+// every load this wave issued has returned its data (vmcnt = 0); cheaper than
+// a release fence, which also writes back the L2
+__device__ inline void loads_done() { __builtin_amdgcn_s_waitcnt(0); }
 
 // waits for every move t' < t whose source overlaps [dst, dst+len)
 __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
@@ -151,6 +166,20 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         if (hi >= m.nchunk) hi = m.nchunk - 1;
         for (uint64_t c = lo; c <= hi; c++) {
             const uint64_t tt = (m.kind == MV_SHIFT_RIGHT) ? m.nchunk - 1 - c : c;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+        }
+        return;
+    }
+    if (m.kind == MV_ROW_SLIDE) {
+        // destination keys [dst, dst + rows*l); sources are groups of CH rows
+        const uint64_t dlen = len / (m.l + 1) * m.l;
+        if (dlen == 0) return;
+        const uint64_t G = m.CH * (m.l + 1);
+        const uint64_t glo = (dst - m.P) / G;
+        uint64_t ghi = (dst + dlen - 1 - m.P) / G;
+        if (ghi >= m.nchunk) ghi = m.nchunk - 1;
+        for (uint64_t g = glo; g <= ghi; g++) {
+            const uint64_t tt = m.nchunk - 1 - g;
             if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
         }
         return;
@@ -173,10 +202,38 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
     }
 }
 
+// store of one loaded move: contiguous, or (row slide) keys contiguous and
+// every (l+1)-th element to the stash
+template <typename T, int PER>
+__device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t src, uint64_t dst,
+                                  uint64_t len, const T (&v)[PER]) {
+    if (ms.kind == MV_ROW_SLIDE) {
+        const uint32_t L1 = (uint32_t)(ms.l + 1), l = (uint32_t)ms.l;
+        T *st = stash + (src - ms.P) / L1;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint32_t idx = threadIdx.x + (uint32_t)j * blockDim.x;
+            if (idx < len) {
+                const uint32_t row = idx / L1, c = idx - row * L1;
+                if (c == l)
+                    st[row] = v[j];
+                else
+                    A[dst + (uint64_t)row * l + c] = v[j];
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+        if (idx < len) A[dst + idx] = v[j];
+    }
+}
+
 template <typename T, int PER>
 __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
                                                  uint32_t *flags, unsigned long long *counter,
-                                                 uint32_t *err, uint64_t limit) {
+                                                 uint32_t *err, uint64_t limit, T *stash) {
     __shared__ uint64_t s_t;
     for (;;) {
         if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
@@ -193,18 +250,14 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
             if (idx < len) v[j] = A[src + idx];
         }
         // every wave's loads complete before the chunk is published as read
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        loads_done();
         __syncthreads();
         if (threadIdx.x == 0) {
             __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             wait_deps(ms, t, dst, len, flags, err, limit);
         }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PER; j++) {
-            const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
-            if (idx < len) A[dst + idx] = v[j];
-        }
+        store_move<T, PER>(A, stash, ms, src, dst, len, v);
     }
 }
 

@@ -267,7 +267,7 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
             unsigned long long *counter = (unsigned long long *)(flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
             hipLaunchKernelGGL((k_chained<T, 16>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
-                               flags, counter, c->err_flag, c->wait_ticks);
+                               flags, counter, c->err_flag, c->wait_ticks, stash);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -332,6 +332,7 @@ static tl_status export_plan(Layout lay, uint64_t n, uint64_t b, tl_dtype dt, ui
     uint32_t W;
     TL_TRY(check_args(lay, (void *)1, n, b, dt, W));
     Tuning tu;
+    tuning_env(tu);
     if (lds_bytes) tu.lds_bytes = lds_bytes;
     Plan pl;
     build_plan(pl, tu, lay, n, b, W);

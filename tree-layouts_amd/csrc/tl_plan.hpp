@@ -82,10 +82,18 @@ struct Tuning {
     size_t lds_bytes = 65536;   // LDS budget of one in-LDS layout workgroup
     uint64_t fmax = 256;        // largest merge radix
     uint64_t stash_max = 1ull << 24;   // elements a rotation may stash
+    uint64_t row_stash_max = 1ull << 24;   // separators a row slide may stash
+    int64_t row_stash_env = -1;            // $TL_ROW_STASH_MAX when set (0: no row slides)
 };
+
+inline void tuning_env(Tuning &t) {
+    const char *r = getenv("TL_ROW_STASH_MAX");
+    if (r) t.row_stash_env = atoll(r);
+}
 
 inline Tuning tuning_for(Ctx *c) {
     Tuning t;
+    tuning_env(t);
     const char *e = getenv("TL_LDS_BYTES");
     size_t want = e ? (size_t)atoll(e) : 163840;
     // the in-LDS kernel keeps 512 bytes of static LDS beside the dynamic area
@@ -232,12 +240,35 @@ struct Planner {
 
     // ---- dense gather: every item of `runs` holds M rows of b leaf keys and
     //      one separator -> [M separators][M*b leaf keys]
+    // one chained pass over r rows of (l keys, 1 separator) at P: separators
+    // to the stash, keys slid right into place, separators back to the front
+    void row_slide(uint64_t P, uint64_t r, uint64_t l) {
+        Op op;
+        op.type = OP_CHAINED;
+        op.ms.kind = MV_ROW_SLIDE;
+        op.ms.P = P;
+        op.ms.r = r;
+        op.ms.l = l;
+        op.ms.CH = std::max<uint64_t>(1, (256 * 16) / (l + 1));
+        op.ms.nchunk = (r + op.ms.CH - 1) / op.ms.CH;
+        op.p[0] = op.ms.nchunk;
+        pl.max_moves = std::max(pl.max_moves, op.p[0]);
+        pl.max_stash = std::max(pl.max_stash, r);
+        pl.ops.push_back(op);
+        copy(P, 1, r, /*to_stash=*/false);
+    }
+
     void gather_dense(const std::vector<Run> &runs, uint64_t M, uint64_t b) {
         if (runs.empty() || M == 0) return;
         const uint64_t k1 = b + 1;
         // a single row always goes to LDS, whatever the budget
         if (M * k1 <= cap() || M == 1) {
             lds(runs, M * k1, LDS_GATHER, b, M);
+            return;
+        }
+        // one large region whose separators fit the stash: one chained pass
+        if (runs.size() == 1 && runs[0].count == 1 && k1 <= 2048 && M <= tu.row_stash_max) {
+            row_slide(runs[0].start, M, b);
             return;
         }
         const uint64_t cap_r = std::max<uint64_t>(1, cap() / k1);
@@ -511,6 +542,12 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             if (m.kind == MV_SPARSE_SLIDE) {
                 end = m.P + m.r * (m.l + 1);
                 if (m.CH == 0 || m.CH > 256 * 16 || m.CH > m.l) neg = true;
+            } else if (m.kind == MV_ROW_SLIDE) {
+                end = m.P + m.r * (m.l + 1);
+                if (m.CH == 0 || m.CH * (m.l + 1) > 256 * 16 || m.nchunk * m.CH < m.r ||
+                    m.r > pl.max_stash || op.p[0] != m.nchunk)
+                    neg = true;
+                break;
             } else if (m.kind == MV_SHIFT_RIGHT) {
                 end = m.P + m.L + m.k;
             } else {
@@ -543,7 +580,11 @@ inline bool width_of(tl_dtype dt, uint32_t &W) {
 
 enum Layout { L_BST, L_BTREE, L_MVEB };
 
-inline void build_plan(Plan &pl, const Tuning &tu, Layout lay, uint64_t n, uint64_t b, uint32_t W) {
+inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint64_t b, uint32_t W) {
+    // scratch stays at most n/16 keys (or 2^24 keys for small arrays)
+    Tuning tu = tu0;
+    tu.row_stash_max = tu.row_stash_env >= 0 ? (uint64_t)tu.row_stash_env
+                                              : std::max<uint64_t>(tu.stash_max, n / 16);
     Planner P{pl, tu, W};
     if (lay == L_BST)
         plan_bst(P, n);
