@@ -191,6 +191,87 @@ def run_batch(args, dev):
     B.close()
 
 
+def run_layout(args, dev, layout):
+    """Level-order BST or mvEB at n = 2^log2n uint64 (same key generator as the
+    headline): in-place permute timing and plain query timing."""
+    import torch
+    import tl_amd
+    n, nq = 1 << args.log2n, 1 << args.log2q
+    rng = np.random.default_rng(4)
+    log(f"[bench] {layout}: generating n={n} keys and {nq} queries")
+    keys, g = gen_keys(n, 4, dev)
+    qidx = torch.randint(0, n, (nq,), generator=g, device=dev, dtype=torch.int64)
+    Q = keys[qidx]
+    del qidx
+    A = keys.clone()
+    stream = torch.cuda.current_stream()
+    tl_amd.permute(layout, A, kind="u64")
+    torch.cuda.synchronize()
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    o = ol.oracle()
+    ranks = rng.integers(0, n, size=100_000, dtype=np.int64)
+    if layout == "bst":
+        pos = np.array([o.or_btree_pos(int(x), n, 1) for x in ranks], dtype=np.int64)
+    else:
+        pos = np.array([o.or_mveb_pos(int(x), n) for x in ranks], dtype=np.int64)
+    got = A[torch.from_numpy(pos).to(dev)].cpu().numpy()
+    layout_ok = bool(np.array_equal(got, keys[torch.from_numpy(ranks).to(dev)].cpu().numpy()))
+    log(f"[bench] layout sample check: {layout_ok}")
+    for _ in range(args.warmup):
+        tl_amd.permute(layout, A, kind="u64", timed=False)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        tl_amd.permute(layout, A, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    perm_ms = ev0.elapsed_time(ev1) / args.steps
+    log(f"[bench] permute {perm_ms:.3f} ms/step")
+    A.copy_(keys)
+    tl_amd.permute(layout, A, kind="u64")
+    ans = torch.empty(nq, dtype=torch.int64, device=dev)
+    if layout == "bst":
+        def q():
+            tl_amd.query_bst(A, Q, ans, kind="u64", timed=False)
+    else:
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n)
+        num_tables = int(meta[0].item())
+        d = int(np.floor(np.log2(n))) + 1
+
+        def q():
+            tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind="u64",
+                                         timed=False)
+    q()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        q()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    q_ms = ev0.elapsed_time(ev1) / args.steps
+    ok = bool(torch.equal(A[ans.clamp(0, n - 1)], Q)) and bool(((ans >= 0) & (ans < n)).all())
+    log(f"[bench] query {q_ms:.3f} ms ({nq / q_ms / 1e3:.1f} Mq/s) ok={ok}")
+    achieved = 2.0 * n * 8 / (perm_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
+        "value": round(n / (perm_ms * 1e-3) / 1e9, 3), "unit": "Gkeys/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(perm_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic: lognormal(0,2) gaps, torch CUDA generator seed 4",
+        "config": {"workload": f"level-order {layout}, in-place permute of n=2^{args.log2n} uint64 "
+                               f"keys; 2^{args.log2q} uniform-hit queries", "n": n, "queries": nq},
+        "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2)},
+        "checks": {"layout_sample": layout_ok, "answers": ok},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"whole tl_permute_{layout} call (all its launches)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,7 +282,7 @@ def main():
     ap.add_argument("--b", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--workload", choices=["btree", "batch"], default="btree")
+    ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb"], default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -214,6 +295,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "batch":
         return run_batch(args, dev)
+    if args.workload in ("bst", "mveb"):
+        return run_layout(args, dev, args.workload)
     n, nq, b = 1 << args.log2n, 1 << args.log2q, args.b
     rng = np.random.default_rng(4)
     log(f"[bench] generating n={n} keys and {nq} queries")

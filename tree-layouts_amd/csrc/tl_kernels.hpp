@@ -26,19 +26,36 @@ __device__ inline uint32_t rev_low_bits32(uint32_t x, uint32_t bits) {
 //     (2^K rows of 2^K consecutive keys) exchanges with tile rev(m) through
 //     LDS, transposed.  A key whose index base + y is negative does not exist
 //     (the 1-based BST index 0).  Carter-Gatlin style blocking.
+//     Tile numbers are visited so that consecutive workgroups share the pages
+//     of both tiles of their pair: with 2^p tiles per 2 MiB page, the tile
+//     number's middle 2p - MB bits (which change neither page) vary fastest.
+__device__ inline uint64_t bitrev_tile_order(uint64_t t, uint32_t MB, uint32_t p) {
+    const uint32_t c = (2 * p > MB) ? min(2 * p - MB, MB) : 0;
+    const uint32_t sb = (MB - c) / 2;
+    const uint64_t C = t & ((1ull << c) - 1);
+    const uint64_t B = (t >> c) & ((1ull << sb) - 1);
+    const uint64_t Ahi = t >> (c + sb);
+    return (Ahi << (sb + c)) | (C << sb) | B;
+}
+
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
                                                       uint32_t D) {
     constexpr int R = 1 << K;
+    constexpr int E = R * R / 256;   // elements per thread and tile
     __shared__ T sA[R][R + 1];
     __shared__ T sB[R][R + 1];
     const uint32_t MB = D - 2 * K;
-    const uint64_t m = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
-    if (MB < 64 && (m >> MB) != 0) return;
+    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (MB < 64 && (t >> MB) != 0) return;
+    constexpr uint32_t row_log2 = (uint32_t)K + (sizeof(T) == 8 ? 3u : 2u);
+    const uint64_t m = bitrev_tile_order(t, MB, 21u - row_log2);
     const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
     if (m > mr) return;
     const uint32_t hs = D - K;
-    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint32_t e = threadIdx.x + k * 256;
         const uint32_t h = e >> K, l = e & (R - 1);
         const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
         if (ia >= 0) sA[h][l] = A[ia];

@@ -31,20 +31,36 @@
 
 namespace tl {
 
-// in-LDS layout over runs of items, `per` items per workgroup
+// in-LDS layout over runs of items, `per` items per workgroup.  With several
+// items per workgroup the slot of every in-item index is computed once into
+// an LDS table behind the keys; i / n uses a multiply by ceil(2^32 / n),
+// exact because i * n < 2^32.
+__device__ inline uint32_t lds_slot(uint32_t x, uint32_t n, uint32_t kind, uint32_t b,
+                                    uint32_t rows, const BtreeShape &sh) {
+    if (kind == LDS_BTREE) return btree_pos32(x, sh);
+    if (kind == LDS_MVEB) return mveb_pos32(x, n);
+    if (kind == LDS_GATHER) return gather_pos32(x, rows, b);
+    const uint32_t y = x + 1, t = 31u - __clz(y);
+    return ((1u << t) | rev_low_bits32(y & ((1u << t) - 1), t)) - 1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                                   uint32_t nruns, uint64_t nitems, uint32_t per,
                                                   uint32_t n, uint32_t kind, uint32_t b,
-                                                  uint32_t rows) {
+                                                  uint32_t rows, uint32_t magic) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *s = reinterpret_cast<T *>(smem_raw);
+    uint32_t *tbl = reinterpret_cast<uint32_t *>(smem_raw + (size_t)per * n * sizeof(T));
     __shared__ uint64_t s_off[64];
     BtreeShape sh;
     if (kind == LDS_BTREE) sh = btree_shape(n, b);
     const uint64_t first = (uint64_t)blockIdx.x * per;
     const uint32_t cnt = (uint32_t)min((uint64_t)per, nitems - first);
+    const bool table = per > 1;
     for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) s_off[t] = run_offset(runs, nruns, first + t);
+    if (table)
+        for (uint32_t x = threadIdx.x; x < n; x += blockDim.x) tbl[x] = lds_slot(x, n, kind, b, rows, sh);
     __syncthreads();
     const uint32_t tot = cnt * n;
     constexpr int U = 8;
@@ -55,7 +71,7 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
         for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * blockDim.x;
             if (i < tot) {
-                it[u] = i / n;
+                it[u] = table ? __umulhi(i, magic) : 0;
                 ii[u] = i - it[u] * n;
                 v[u] = A[s_off[it[u]] + ii[u]];
             }
@@ -64,41 +80,49 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
         for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * blockDim.x;
             if (i < tot) {
-                const uint32_t x = ii[u];
-                uint32_t p;
-                if (kind == LDS_BTREE)
-                    p = btree_pos32(x, sh);
-                else if (kind == LDS_MVEB)
-                    p = mveb_pos32(x, n);
-                else if (kind == LDS_GATHER)
-                    p = gather_pos32(x, rows, b);
-                else {
-                    const uint32_t y = x + 1, t = 31u - __clz(y);
-                    p = ((1u << t) | rev_low_bits32(y & ((1u << t) - 1), t)) - 1;
-                }
+                const uint32_t p = table ? tbl[ii[u]] : lds_slot(ii[u], n, kind, b, rows, sh);
                 s[it[u] * n + p] = v[u];
             }
         }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
-        const uint32_t t = i / n, x = i - t * n;
+        const uint32_t t = table ? __umulhi(i, magic) : 0, x = i - t * n;
         A[s_off[t] + x] = s[i];
     }
 }
 
+// reverse every segment [off, off+len): REV_U mirrored pairs per thread per
+// step, all loads issued before any store (the pairs are disjoint)
+constexpr int REV_U = 8;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_reverse_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                                       uint32_t nruns, uint64_t nitems, uint64_t len) {
+    const uint64_t half = len / 2;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * REV_U;
     for (uint64_t seg = blockIdx.y; seg < nitems; seg += gridDim.y) {
-        const uint64_t off = run_offset(runs, nruns, seg);
-        const uint64_t half = len / 2;
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half;
-             i += (uint64_t)gridDim.x * blockDim.x) {
-            T x = A[off + i];
-            T y = A[off + len - 1 - i];
-            A[off + i] = y;
-            A[off + len - 1 - i] = x;
+        T *a = A + run_offset(runs, nruns, seg);
+        T *z = a + len - 1;
+        for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * REV_U + threadIdx.x; base < half;
+             base += step) {
+            T x[REV_U], y[REV_U];
+#pragma unroll
+            for (int j = 0; j < REV_U; j++) {
+                const uint64_t i = base + (uint64_t)j * blockDim.x;
+                if (i < half) {
+                    x[j] = a[i];
+                    y[j] = *(z - i);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < REV_U; j++) {
+                const uint64_t i = base + (uint64_t)j * blockDim.x;
+                if (i < half) {
+                    a[i] = y[j];
+                    *(z - i) = x[j];
+                }
+            }
         }
     }
 }
@@ -197,11 +221,13 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
         case OP_LDS: {
             const uint64_t n = op.p[0], per = op.p[4];
             const uint64_t groups = (op.nitems + per - 1) / per;
-            const size_t bytes = per * n * sizeof(T);
+            const size_t bytes = per * n * sizeof(T) + (per > 1 ? n * 4 : 0);
             const int threads = (per * n >= 8192) ? 512 : 256;
+            // ceil(2^32 / n); per * n < 2^16 whenever several items share a workgroup
+            const uint32_t magic = (uint32_t)((0xFFFFFFFFull + n) / n);
             hipLaunchKernelGGL(k_lds_runs<T>, dim3((uint32_t)groups), dim3(threads), bytes, s, A,
                                runs, (uint32_t)op.nruns, op.nitems, (uint32_t)per, (uint32_t)n,
-                               (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3]);
+                               (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3], magic);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -209,7 +235,7 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
             const uint64_t len = op.p[0];
             if (len < 2) break;
             const uint64_t half = len / 2;
-            uint64_t bx = (half + 255) / 256;
+            uint64_t bx = (half + 256 * REV_U - 1) / (256 * REV_U);
             bx = std::min<uint64_t>(bx, (uint64_t)c->num_cus * 16);
             const uint64_t by = std::min<uint64_t>(op.nitems, 65535);
             hipLaunchKernelGGL(k_reverse_runs<T>, dim3((uint32_t)bx, (uint32_t)by), dim3(256), 0,

@@ -84,6 +84,7 @@ struct Tuning {
     uint64_t stash_max = 1ull << 24;   // elements a rotation may stash
     uint64_t row_stash_max = 1ull << 24;   // separators a row slide may stash
     int64_t row_stash_env = -1;            // $TL_ROW_STASH_MAX when set (0: no row slides)
+    uint64_t pack_bytes = 32768;           // keys of several small in-LDS items per workgroup
 };
 
 inline void tuning_env(Tuning &t) {
@@ -157,7 +158,10 @@ struct Planner {
         op.type = OP_LDS;
         pl.put_runs(op, runs);
         if (op.nitems == 0) return;
-        uint64_t per = std::max<uint64_t>(1, cap() / n);
+        // small items share a workgroup up to tu.pack_bytes of keys, so that
+        // several workgroups fit a compute unit
+        const uint64_t pack = std::min<uint64_t>(tu.lds_bytes, tu.pack_bytes) / W;
+        uint64_t per = std::max<uint64_t>(1, pack / n);
         if (per > 64) per = 64;
         op.p[0] = n;
         op.p[1] = kind;
