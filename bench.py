@@ -64,6 +64,19 @@ def sample_check_layout(A, keys_sorted, n, b, rng, samples=200_000):
     return bool(np.array_equal(got, exp))
 
 
+def measured_traffic(name):
+    """HBM bytes per permute call from a committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("Total per call:"):
+                return float(line.split("=")[1].split("GB")[0]) * 1e9
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(keys_np, b, nq_sample, rng):
     """Oracle OpenMP restatement of the reference CPU path (btree-cycles +
     query-btree), on a bounded sample, on the host cores."""
@@ -360,6 +373,9 @@ def main():
 
     algo_bytes = 2.0 * n * 8
     achieved = algo_bytes / (perm_ms * 1e-3) / 1e9
+    traffic = None
+    if n == 1 << 30 and b == 16:
+        traffic = measured_traffic("btree_b16_n2p30_u64_permute_traffic.txt")
     out = {
         "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
         "value": round(n / (perm_ms * 1e-3) / 1e9, 3),
@@ -381,8 +397,11 @@ def main():
         "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "whole tl_permute_btree call (all its launches)"},
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_GBs": None if traffic is None
+                     else round(traffic / (perm_ms * 1e-3) / 1e9, 1),
+                     "kernel": "one tl_permute_btree call (all its launches); traffic from "
+                               "profiles/btree_b16_n2p30_u64_permute_traffic.txt"},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["permute_gkeys_s"], 4), "unit": "Gkeys/s", "cores": cpu["cores"],
             "kind": "port",
