@@ -295,6 +295,7 @@ def main():
     ap.add_argument("--b", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--log2leaves", type=int, default=20, help="learned model leaves (log2)")
     ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb"], default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     args = ap.parse_args()
@@ -361,7 +362,8 @@ def main():
 
     learned = None
     if not args.skip_learned and hasattr(tl_amd, "learned_fit"):
-        learned = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n)
+        learned = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n,
+                                       num_leaves=1 << args.log2leaves)
 
     cpu = None
     if not args.no_cpu:

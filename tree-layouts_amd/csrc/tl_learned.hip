@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "tl_internal.hpp"
+#include "tl_descent.hpp"
 
 namespace tl {
 
@@ -204,118 +205,154 @@ __global__ __launch_bounds__(256) void k_leaf_fit(const K *__restrict__ keys,
 // ---------------------------------------------------------------------------
 // query
 
-struct BtreeConst {   // rank -> slot closed form of the searched layout
+// Start node of a rank window [lo, hi] in the level-order B-tree, in closed
+// form.  A rank is placed among the internal tree's keys: internal rank r sits
+// at (p, q) = (r, r+1) and every key of leaf node j at (j, j) (between
+// internal ranks j-1 and j).  The internal node at height h (bottom internal
+// level h = 1) with block m holds exactly the positions with
+// floor(p / k1^h) = floor(q / k1^h) = m, so the lowest node holding the whole
+// window is the first height where floor(p_lo / k1^h) = floor(q_hi / k1^h).
+// Both keys in one leaf node start at that leaf node.
+struct BtreeConst {
     uint64_t n, b, k1;
     uint64_t perfect, nint, span, rem, nsep;
+    uint64_t mk1;              // ceil(2^64 / k1)
+    uint32_t D, pad;           // levels of the internal (whole, if perfect) tree
+    uint64_t pw[41];           // k1^h
+    uint64_t mag[41];          // ceil(2^64 / k1^h)
+    uint64_t nodestart[41];    // heap index of the first node at height h
 };
 
-__device__ inline uint64_t slot_of_rank(uint64_t x, const BtreeConst &c) {
-    uint64_t n = c.n;
+// x / k1^h.  With FAST (x < 2^32, k1^h < 2^32) a multiply-high by
+// ceil(2^64 / k1^h) is exact: x * e < 2^64 for the rounding excess e < k1^h.
+template <bool FAST>
+__device__ inline uint64_t div_pw(uint64_t x, uint32_t h, const uint64_t *pw, const uint64_t *mag) {
+    return FAST ? __umul64hi(x, mag[h]) : x / pw[h];
+}
+
+template <bool FAST>
+__device__ inline void rank_place(uint64_t x, const BtreeConst &c, const uint64_t *pw,
+                                  const uint64_t *mag, uint64_t &p, uint64_t &q, bool &leaf) {
+    leaf = false;
     if (!c.perfect) {
         if (x < c.span) {
-            const uint64_t row = x / c.k1, cc = x - row * c.k1;
-            if (cc < c.b) return c.nint + row * c.b + cc;
-            x = row;
-        } else if (x < c.span + c.rem) {
-            return c.nint + (c.span - c.nsep) + (x - c.span);
-        } else {
-            x = c.nsep + (x - c.span - c.rem);
+            const uint64_t row = div_pw<FAST>(x, 1, pw, mag), cc = x - row * c.k1;
+            p = row;
+            q = cc < c.b ? row : row + 1;
+            leaf = cc < c.b;
+            return;
         }
-        n = c.nint;
+        if (x < c.span + c.rem) {
+            p = q = c.nsep;
+            leaf = true;
+            return;
+        }
+        x = c.nsep + (x - c.span - c.rem);
     }
-    uint64_t r = x + 1, div = 1, q = r / c.k1;
-    while (r == q * c.k1) {
-        r = q;
-        div *= c.k1;
-        q = r / c.k1;
-    }
-    const uint64_t a = r - q * c.k1;
-    return (n + 1) / (div * c.k1) - 1 + q * c.b + a - 1;
+    p = x;
+    q = x + 1;
 }
 
-template <typename K, int G>
-__device__ inline uint64_t group_descent(const K *__restrict__ A, uint64_t n, uint32_t b, K q,
-                                         uint64_t i, bool active, uint32_t g, uint32_t j,
-                                         bool &found) {
-    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1);
-    uint64_t res = n;
-    found = false;
-    while (__any(active)) {
-        bool lt = false, eq = false;
-        if (active && j < b && i + j < n) {
-            const K key = A[i + j];
-            lt = key < q;
-            eq = key == q;
-        }
-        const uint64_t mlt = (__ballot(lt) >> (g * G)) & gmask;
-        const uint64_t meq = (__ballot(eq) >> (g * G)) & gmask;
-        if (active) {
-            const uint32_t js = (uint32_t)__popcll(mlt);
-            if ((meq >> js) & 1ull) {
-                res = i + js;
-                found = true;
-                active = false;
-            } else {
-                i = (i + 1) * (b + 1) + (uint64_t)js * b - 1;
-                if (i >= n) active = false;
-            }
-        }
-    }
-    return res;
+template <bool FAST>
+__device__ inline uint64_t start_slot(uint64_t lo, uint64_t hi, const BtreeConst &c,
+                                      const uint64_t *pw, const uint64_t *mag,
+                                      const uint64_t *nodestart) {
+    uint64_t plo, qlo, phi, qhi;
+    bool llo, lhi;
+    rank_place<FAST>(lo, c, pw, mag, plo, qlo, llo);
+    rank_place<FAST>(hi, c, pw, mag, phi, qhi, lhi);
+    if (llo && lhi && plo == phi) return c.nint + plo * c.b;   // one leaf node
+    uint32_t h = 1;
+    while (h < c.D && div_pw<FAST>(plo, h, pw, mag) != div_pw<FAST>(qhi, h, pw, mag)) h++;
+    return (nodestart[h] + div_pw<FAST>(plo, h, pw, mag)) * c.b;
 }
 
-template <typename K, int G>
-__global__ __launch_bounds__(256) void k_query_btree_learned(
-    const K *__restrict__ A, BtreeConst bc, const LearnedHeader *__restrict__ hp,
-    const LeafModel *__restrict__ leaves, const K *__restrict__ Q, uint64_t nq,
-    uint64_t *__restrict__ ans, unsigned long long *stats) {
+// Pass 1: the start slot of every query, one lane per query, written into the
+// answer array (no dependent chain, so the model loads and the index
+// arithmetic run at throughput).  Queries without a usable leaf start at the
+// root (slot 0) and are counted as fallbacks.
+template <typename K, bool FAST>
+__global__ __launch_bounds__(256) void k_learned_start(BtreeConst bc,
+                                                       const LearnedHeader *__restrict__ hp,
+                                                       const LeafModel *__restrict__ leaves,
+                                                       const K *__restrict__ Q, uint64_t nq,
+                                                       uint64_t *__restrict__ ans,
+                                                       unsigned long long *stats) {
+    __shared__ uint64_t s_pw[41], s_mag[41], s_ns[41];
+    for (uint32_t t = threadIdx.x; t < 41; t += blockDim.x) {
+        s_pw[t] = bc.pw[t];
+        s_mag[t] = bc.mag[t];
+        s_ns[t] = bc.nodestart[t];
+    }
+    __syncthreads();
     const LearnedHeader h = *hp;
-    const uint64_t n = bc.n;
-    const uint32_t b = (uint32_t)bc.b;
+    uint32_t fb = 0;
+    for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < nq;
+         qi += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t u = ukey<K>(Q[qi]);
+        const LeafModel m = leaves[leaf_of(u, h)];
+        uint64_t start = 0;
+        if (m.count) {
+            const int64_t pred = predict(u, m);
+            int64_t lo = pred - (int64_t)m.err, hi = pred + (int64_t)m.err;
+            lo = max(lo, (int64_t)m.base_rank);
+            hi = min(hi, (int64_t)(m.base_rank + m.count - 1));
+            if (lo <= hi)
+                start = start_slot<FAST>((uint64_t)lo, (uint64_t)hi, bc, s_pw, s_mag, s_ns);
+            else
+                fb++;
+        } else {
+            fb++;
+        }
+        ans[qi] = start;
+    }
+    if (stats) {
+        for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
+        if ((threadIdx.x & 63) == 0 && fb) atomicAdd(&stats[0], (unsigned long long)fb);
+    }
+}
+
+// Pass 2: group descent from each query's start slot (read from the answer
+// array), two queries per group.  A descent that started below the root and
+// did not find the key is redone from the root: the plain descent decides
+// every other case, so the answers equal tl_query_btree's.
+template <typename K, int G>
+__global__ __launch_bounds__(256) void k_query_btree_from(const K *__restrict__ A, uint64_t n,
+                                                          uint32_t b, const K *__restrict__ Q,
+                                                          uint64_t nq, uint64_t *__restrict__ ans,
+                                                          unsigned long long *stats) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t g = lane / G, j = lane % G;
     const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
-    const uint64_t trips = (nq + groups - 1) / groups;
-    uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    const uint64_t trips = (nq + 2 * groups - 1) / (2 * groups);   // uniform trip count
+    uint64_t qa = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
     uint32_t fb = 0;
-    for (uint64_t it = 0; it < trips; it++, qi += groups) {
-        const bool valid = qi < nq;
-        K q{};
-        if (valid) q = Q[qi];
-        // start node from the model (uniform inside the group)
-        uint64_t start = 0;
-        bool use_model = false;
-        if (valid) {
-            const uint64_t u = ukey<K>(q);
-            const LeafModel m = leaves[leaf_of(u, h)];
-            if (m.count) {
-                const int64_t pred = predict(u, m);
-                int64_t lo = pred - (int64_t)m.err, hi = pred + (int64_t)m.err;
-                lo = max(lo, (int64_t)m.base_rank);
-                hi = min(hi, (int64_t)(m.base_rank + m.count - 1));
-                if (lo <= hi) {
-                    uint64_t ua = slot_of_rank((uint64_t)lo, bc) / b;
-                    uint64_t ub = slot_of_rank((uint64_t)hi, bc) / b;
-                    while (ua != ub) {   // deeper node has the larger heap index
-                        if (ua > ub)
-                            ua = (ua - 1) / (b + 1);
-                        else
-                            ub = (ub - 1) / (b + 1);
-                    }
-                    start = ua * b;
-                    use_model = true;
-                }
-            }
+    for (uint64_t it = 0; it < trips; it++, qa += 2 * groups) {
+        const uint64_t qb = qa + groups;
+        const bool va = qa < nq, vb = qb < nq;
+        K q0{}, q1{};
+        uint64_t s0 = 0, s1 = 0;
+        if (va) {
+            q0 = Q[qa];
+            s0 = ans[qa];
         }
-        bool found;
-        uint64_t res = group_descent<K, G>(A, n, b, q, start, valid, g, j, found);
-        // a descent that started below the root and did not find the key is
-        // redone from the root: the plain descent decides every other case
-        const bool again = valid && start != 0 && !found;
-        if (valid && j == 0 && (again || !use_model)) fb++;
-        const uint64_t res2 = group_descent<K, G>(A, n, b, q, 0, again, g, j, found);
-        if (again) res = res2;
-        if (valid && j == 0) ans[qi] = res;
+        if (vb) {
+            q1 = Q[qb];
+            s1 = ans[qb];
+        }
+        uint64_t r0, r1;
+        bool f0, f1;
+        descent2<K, G>(A, n, b, g, j, q0, q1, s0, s1, va, vb, r0, r1, f0, f1);
+        const bool again0 = va && s0 != 0 && !f0, again1 = vb && s1 != 0 && !f1;
+        if (j == 0) fb += again0 + again1;
+        uint64_t p0, p1;
+        descent2<K, G>(A, n, b, g, j, q0, q1, 0, 0, again0, again1, p0, p1, f0, f1);
+        if (again0) r0 = p0;
+        if (again1) r1 = p1;
+        if (j == 0) {
+            if (va) ans[qa] = r0;
+            if (vb) ans[qb] = r1;
+        }
     }
     if (stats) {
         for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
@@ -382,16 +419,40 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
         bc.rem = rem;
         bc.nsep = i;
     }
+    // ceil(2^64 / d) = floor((2^64 - 1) / d) + 1 for every d > 1
+    auto ceil_recip = [](uint64_t d) -> uint64_t { return ~0ull / d + 1; };
+    bc.mk1 = ceil_recip(bc.k1);
+    bc.D = bc.perfect ? h + 1 : h;   // levels of the internal (or whole) tree
+    {
+        uint64_t p = 1;
+        for (uint32_t t = 0; t <= 40; t++) {
+            bc.pw[t] = p;
+            bc.mag[t] = t ? ceil_recip(p) : 0;
+            if (t >= 1 && t <= bc.D) bc.nodestart[t] = (ipow64(bc.k1, bc.D - t) - 1) / b;
+            if (p > (~0ull) / bc.k1) p = ~0ull;   // saturate; never reached below height D
+            else p *= bc.k1;
+        }
+    }
+    const bool fast = n < (1ull << 32);
     const LearnedHeader *hp = (const LearnedHeader *)model;
     const LeafModel *leaves = (const LeafModel *)((const char *)model + sizeof(LearnedHeader));
+    const uint64_t sgrid = std::max<uint64_t>(1, std::min<uint64_t>((nq + 255) / 256,
+                                                                     (uint64_t)c->num_cus * 16));
+    if (fast)
+        hipLaunchKernelGGL((k_learned_start<K, true>), dim3((uint32_t)sgrid), dim3(256), 0, s, bc, hp,
+                           leaves, (const K *)q, nq, ans, stats);
+    else
+        hipLaunchKernelGGL((k_learned_start<K, false>), dim3((uint32_t)sgrid), dim3(256), 0, s, bc,
+                           hp, leaves, (const K *)q, nq, ans, stats);
+    TL_LAUNCH_CHECK();
     const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
-    const uint64_t want = (nq + 256 / G - 1) / (256 / G);
+    const uint64_t want = (nq + 2 * 256 / G - 1) / (2 * 256 / G);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));
     switch (G) {
 #define TL_G(GG)                                                                                    \
     case GG:                                                                                        \
-        hipLaunchKernelGGL((k_query_btree_learned<K, GG>), dim3(grid), dim3(256), 0, s,             \
-                           (const K *)A, bc, hp, leaves, (const K *)q, nq, ans, stats);             \
+        hipLaunchKernelGGL((k_query_btree_from<K, GG>), dim3(grid), dim3(256), 0, s, (const K *)A,  \
+                           n, (uint32_t)b, (const K *)q, nq, ans, stats);                           \
         break;
         TL_G(2) TL_G(4) TL_G(8) TL_G(16) TL_G(32) TL_G(64)
 #undef TL_G

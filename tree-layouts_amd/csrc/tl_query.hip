@@ -9,6 +9,7 @@
 #include <algorithm>
 
 #include "tl_internal.hpp"
+#include "tl_descent.hpp"
 
 namespace tl {
 
@@ -71,37 +72,22 @@ __global__ __launch_bounds__(256) void k_query_btree_group(const K *__restrict__
                                                            uint64_t nq, uint64_t *__restrict__ ans) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t g = lane / G, j = lane % G;
-    const uint64_t gmask = (G == 64) ? ~0ull : ((1ull << G) - 1);
     const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
-    const uint64_t waves_q = (nq + groups - 1) / groups;   // uniform trip count
-    uint64_t qi = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
-    for (uint64_t it = 0; it < waves_q; it++, qi += groups) {
-        const bool valid = qi < nq;
-        K q{};
-        if (valid) q = Q[qi];
-        uint64_t i = 0, res = n;
-        bool active = valid;
-        while (__any(active)) {
-            bool lt = false, eq = false;
-            if (active && j < b && i + j < n) {
-                const K key = A[i + j];
-                lt = key < q;
-                eq = key == q;
-            }
-            const uint64_t mlt = (__ballot(lt) >> (g * G)) & gmask;
-            const uint64_t meq = (__ballot(eq) >> (g * G)) & gmask;
-            if (active) {
-                const uint32_t js = (uint32_t)__popcll(mlt);
-                if ((meq >> js) & 1ull) {
-                    res = i + js;
-                    active = false;
-                } else {
-                    i = (i + 1) * (b + 1) + (uint64_t)js * b - 1;
-                    if (i >= n) active = false;
-                }
-            }
+    const uint64_t trips = (nq + 2 * groups - 1) / (2 * groups);   // uniform trip count
+    uint64_t qa = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    for (uint64_t it = 0; it < trips; it++, qa += 2 * groups) {
+        const uint64_t qb = qa + groups;
+        const bool va = qa < nq, vb = qb < nq;
+        K q0{}, q1{};
+        if (va) q0 = Q[qa];
+        if (vb) q1 = Q[qb];
+        uint64_t r0, r1;
+        bool f0, f1;
+        descent2<K, G>(A, n, b, g, j, q0, q1, 0, 0, va, vb, r0, r1, f0, f1);
+        if (j == 0) {
+            if (va) ans[qa] = r0;
+            if (vb) ans[qb] = r1;
         }
-        if (valid && j == 0) ans[qi] = res;
     }
 }
 
@@ -332,7 +318,7 @@ static void launch_btree(Ctx *c, const void *A, uint64_t n, uint64_t b, const vo
                          uint64_t *ans, hipStream_t s) {
     if (b >= 2 && b <= 64) {
         const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
-        const uint32_t grid = grid_for(c, nq, 256 / G);
+        const uint32_t grid = grid_for(c, nq, 2 * 256 / G);
         switch (G) {
 #define TL_G(GG)                                                                                   \
     case GG:                                                                                       \
