@@ -6,6 +6,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "tree-layouts_amd", "python"))
+sys.path.insert(1, ROOT)   # bench.py's seeded key generator
 
 
 def pytest_configure(config):
