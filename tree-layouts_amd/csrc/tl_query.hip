@@ -14,10 +14,11 @@
 namespace tl {
 
 // ---------------------------------------------------------------------------
-// BST: one lane per query.  The top TOP keys (the first levels) are staged in
-// LDS; each lane keeps two queries in flight so that two dependent descents
-// overlap their memory latency.
+// BST: one lane per query.  The top BST_TOP keys (the first levels) are staged
+// in LDS; each lane keeps BST_QPL queries in flight so that independent
+// dependent descents overlap their memory latency.
 constexpr uint32_t BST_TOP = 2047;
+constexpr int BST_QPL = 4;   // queries in flight per lane
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint64_t n,
@@ -27,36 +28,44 @@ __global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint
     const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP, n);
     for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2;
-    for (uint64_t q0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; q0 < nq; q0 += stride) {
-        const bool two = q0 + 1 < nq;
-        const K qa = Q[q0];
-        const K qb = two ? Q[q0 + 1] : qa;
-        uint64_t ia = 0, ib = two ? 0 : n;
-        uint64_t ra = n, rb = n;
-        while (ia < n || ib < n) {
-            K ca, cb;
-            if (ia < n) ca = ia < ntop ? top[ia] : A[ia];
-            if (ib < n) cb = ib < ntop ? top[ib] : A[ib];
-            if (ia < n) {
-                if (qa == ca) {
-                    ra = ia;
-                    ia = n;
-                } else {
-                    ia = 2 * ia + 1 + (qa > ca);
+    // BST_QPL independent queries per lane, strided by the grid for coalescing
+    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
+         q0 += all * BST_QPL) {
+        K qv[BST_QPL];
+        uint64_t iv[BST_QPL], rv[BST_QPL];
+#pragma unroll
+        for (int k = 0; k < BST_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            qv[k] = qi < nq ? Q[qi] : K{};
+            iv[k] = qi < nq ? 0 : n;
+            rv[k] = n;
+        }
+        bool any = true;
+        while (any) {
+            K cv[BST_QPL];
+#pragma unroll
+            for (int k = 0; k < BST_QPL; k++)
+                if (iv[k] < n) cv[k] = iv[k] < ntop ? top[iv[k]] : A[iv[k]];
+            any = false;
+#pragma unroll
+            for (int k = 0; k < BST_QPL; k++) {
+                if (iv[k] < n) {
+                    if (qv[k] == cv[k]) {
+                        rv[k] = iv[k];
+                        iv[k] = n;
+                    } else {
+                        iv[k] = 2 * iv[k] + 1 + (qv[k] > cv[k]);
+                    }
                 }
-            }
-            if (ib < n) {
-                if (qb == cb) {
-                    rb = ib;
-                    ib = n;
-                } else {
-                    ib = 2 * ib + 1 + (qb > cb);
-                }
+                any |= iv[k] < n;
             }
         }
-        ans[q0] = ra;
-        if (two) ans[q0 + 1] = rb;
+#pragma unroll
+        for (int k = 0; k < BST_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            if (qi < nq) ans[qi] = rv[k];
+        }
     }
 }
 
@@ -309,7 +318,7 @@ static uint32_t grid_for(Ctx *c, uint64_t work, uint32_t per_block) {
 template <typename K>
 static void launch_bst(Ctx *c, const void *A, uint64_t n, const void *q, uint64_t nq, uint64_t *ans,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_query_bst<K>, dim3(grid_for(c, nq, 512)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_query_bst<K>, dim3(grid_for(c, nq, 256 * BST_QPL)), dim3(256), 0, s,
                        (const K *)A, n, (const K *)q, nq, ans);
 }
 
