@@ -85,11 +85,14 @@ struct Tuning {
     uint64_t row_stash_max = 1ull << 24;   // separators a row slide may stash
     int64_t row_stash_env = -1;            // $TL_ROW_STASH_MAX when set (0: no row slides)
     uint64_t pack_bytes = 32768;           // keys of several small in-LDS items per workgroup
+    uint32_t bitrev_k = 0;                 // bit-reversal tile 2^K x 2^K (0: 6 for 4-byte keys, 5 for 8-byte)
 };
 
 inline void tuning_env(Tuning &t) {
     const char *r = getenv("TL_ROW_STASH_MAX");
     if (r) t.row_stash_env = atoll(r);
+    const char *k = getenv("TL_BITREV_K");
+    if (k && (atoi(k) == 5 || atoi(k) == 6)) t.bitrev_k = (uint32_t)atoi(k);
 }
 
 inline Tuning tuning_for(Ctx *c) {
@@ -405,7 +408,7 @@ inline void plan_bst(Planner &P, uint64_t n) {
     } else {
         d = h + 1;
     }
-    const uint32_t K = (P.W == 4) ? 6 : 5;
+    const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : ((P.W == 4) ? 6 : 5);
     if (n <= P.cap() || d < 2 * K + 2) {
         P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
         return;
