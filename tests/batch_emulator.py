@@ -94,10 +94,16 @@ def move_at(m, t):
         j0 = (m.nchunk - 1 - t) * m.CH
         j1 = min(m.r, j0 + m.CH)
         return m.P + j0 * (m.l + 1), m.P + m.r + j0 * m.l, (j1 - j0) * (m.l + 1)
+    ntail = (m.L + m.CH - 1) // m.CH
+    if t < ntail:
+        c = ntail - 1 - t
+        src = m.P + m.r * (m.l + 1) + c * m.CH
+        return src, src + m.k, min(m.CH, m.L - c * m.CH)
+    t -= ntail
     i = m.r - 1 - t // m.nchunk
     c = m.nchunk - 1 - t % m.nchunk
     src = m.P + i * (m.l + 1) + c * m.CH
-    return src, m.P + m.r + i * m.l + c * m.CH, min(m.CH, m.l - c * m.CH)
+    return src, m.P + m.r + m.k + i * m.l + c * m.CH, min(m.CH, m.l - c * m.CH)
 
 
 def awaited_moves(m, t, dst, ln):
@@ -121,6 +127,15 @@ def awaited_moves(m, t, dst, ln):
             if tt < t:
                 out.append(tt)
         return out
+    ntail = (m.L + m.CH - 1) // m.CH
+    T0 = m.P + m.r * (m.l + 1)
+    if m.L and dst + ln > T0 and dst < T0 + m.L:
+        a, z = max(dst, T0) - T0, min(dst + ln, T0 + m.L) - T0
+        for c in range(a // m.CH, (z - 1) // m.CH + 1):
+            if ntail - 1 - c < t:
+                out.append(ntail - 1 - c)
+    if dst >= T0:
+        return out
     L1 = m.l + 1
     rel = dst - m.P
     i_lo, i_hi = rel // L1, min((rel + ln - 1) // L1, m.r - 1)
@@ -131,7 +146,7 @@ def awaited_moves(m, t, dst, ln):
         if ov_end <= bs + ov_lo:
             continue
         for c in range(ov_lo // m.CH, (ov_end - bs - 1) // m.CH + 1):
-            tt = (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c)
+            tt = ntail + (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c)
             if tt < t:
                 out.append(tt)
     return out
@@ -242,14 +257,14 @@ class Replay:
             A[o + r: o + r * (l + 1)] = reg[:, :l].reshape(-1)
             work = r * (l + 1)
         elif t == pe.OP_COPY:
-            P, stride, count, to_stash = p[0], p[1], p[2], p[3]
+            P, stride, count, to_stash, soff = p[0], p[1], p[2], p[3], p[4]
             lo, hi = sub * COPY_CHUNK, min(count, (sub + 1) * COPY_CHUNK)
             st = self.stash.setdefault(op.stash_at, {})
             for i in range(lo, hi):
                 if to_stash:
-                    st[i] = A[P + i * stride]
+                    st[soff + i] = A[P + i * stride]
                 else:
-                    A[P + i * stride] = st[i]
+                    A[P + i * stride] = st[soff + i]
             work = hi - lo
         elif t == pe.OP_CHAINED:
             m = op.ms

@@ -69,7 +69,15 @@ def rev_bits(y, D):
 
 def replay(buf, A):
     ops, blob = decode(buf)
-    stash = None
+    stash = np.zeros(0, dtype=A.dtype)
+
+    def put(off, vals):
+        nonlocal stash
+        if stash.size < off + vals.size:
+            grown = np.zeros(off + vals.size, dtype=A.dtype)
+            grown[:stash.size] = stash
+            stash = grown
+        stash[off:off + vals.size] = vals
     for op in ops:
         t = int(op[0])
         p = [int(v) for v in op[3:11]]
@@ -109,11 +117,11 @@ def replay(buf, A):
                 A[o:o + r] = reg[:, l]
                 A[o + r: o + r * (l + 1)] = reg[:, :l].reshape(-1)
         elif t == OP_COPY:
-            P, stride, count, to_stash = p[0], p[1], p[2], p[3]
+            P, stride, count, to_stash, soff = p[0], p[1], p[2], p[3], p[4]
             if to_stash:
-                stash = A[P: P + count * stride: stride][:count].copy()
+                put(soff, A[P: P + count * stride: stride][:count].copy())
             else:
-                A[P: P + count * stride: stride][:count] = stash[:count]
+                A[P: P + count * stride: stride][:count] = stash[soff:soff + count]
         elif t == OP_CHAINED:
             kind, P, L, k, r, l = (int(v) for v in op[11:17])
             if kind == MV_SHIFT_RIGHT:
@@ -123,11 +131,15 @@ def replay(buf, A):
             elif kind == MV_ROW_SLIDE:
                 # rows of (l keys, 1 separator): separators to the stash
                 reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
-                stash = reg[:, l].copy()
+                put(0, reg[:, l].copy())
                 A[P + r: P + r * (l + 1)] = reg[:, :l].reshape(-1)
             else:
+                # rows of (l leaf keys, 1 root) then a tail of L keys, all slid
+                # right past the r roots and k extra roots (both stashed before)
                 reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
-                A[P + r: P + r * (l + 1)] = reg[:, :l].reshape(-1)
+                tail = A[P + r * (l + 1): P + r * (l + 1) + L].copy()
+                A[P + r + k: P + r + k + r * l] = reg[:, :l].reshape(-1)
+                A[P + r * (l + 1) + k: P + r * (l + 1) + k + L] = tail
         else:
             raise ValueError(f"unknown op {t}")
     return A

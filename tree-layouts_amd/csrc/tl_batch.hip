@@ -226,7 +226,7 @@ template <typename T>
 __device__ void ex_copy(const BOp &op, uint64_t sub, T *stash) {
     T *A = (T *)op.A;
     const uint64_t P = op.p[0], stride = op.p[1], count = op.p[2];
-    T *st = stash + op.stash_at;
+    T *st = stash + op.stash_at + op.p[4];
     const uint64_t lo = sub * COPY_CHUNK, hi = min(count, lo + COPY_CHUNK);
     for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         if (op.p[3])

@@ -122,11 +122,21 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
         len = (j1 - j0) * (m.l + 1);
         dst = m.P + m.r + j0 * m.l;
     } else {
+        // tail chunks first (rightmost first), then the leaf blocks
+        const uint64_t ntail = (m.L + m.CH - 1) / m.CH;
+        if (t < ntail) {
+            const uint64_t c = ntail - 1 - t;
+            src = m.P + m.r * (m.l + 1) + c * m.CH;
+            len = min(m.CH, m.L - c * m.CH);
+            dst = src + m.k;
+            return;
+        }
+        t -= ntail;
         const uint64_t i = m.r - 1 - t / m.nchunk;
         const uint64_t c = m.nchunk - 1 - t % m.nchunk;
         src = m.P + i * (m.l + 1) + c * m.CH;
         len = min(m.CH, m.l - c * m.CH);
-        dst = m.P + m.r + i * m.l + c * m.CH;
+        dst = m.P + m.r + m.k + i * m.l + c * m.CH;
     }
 }
 
@@ -201,7 +211,19 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         }
         return;
     }
-    // sparse slide: sources are the leaf blocks of rows 0..r-1
+    // sparse slide: sources are the tail chunks (numbers 0..ntail-1, the
+    // rightmost first) and the leaf blocks of rows r-1..0 after them
+    const uint64_t ntail = (m.L + m.CH - 1) / m.CH;
+    const uint64_t T0 = m.P + m.r * (m.l + 1);
+    if (m.L && dst + len > T0 && dst < T0 + m.L) {
+        const uint64_t a = (dst > T0 ? dst : T0) - T0;
+        const uint64_t z = (dst + len < T0 + m.L ? dst + len : T0 + m.L) - T0;
+        for (uint64_t c = a / m.CH; c <= (z - 1) / m.CH; c++) {
+            const uint64_t tt = ntail - 1 - c;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+        }
+    }
+    if (dst >= T0) return;
     const uint64_t L1 = m.l + 1;
     const uint64_t rel = dst - m.P;
     uint64_t i_lo = rel / L1, i_hi = (rel + len - 1) / L1;
@@ -213,7 +235,7 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         if (ov_end <= bs + ov_lo) continue;
         const uint64_t c_lo = ov_lo / m.CH, c_hi = (ov_end - bs - 1) / m.CH;
         for (uint64_t c = c_lo; c <= c_hi; c++) {
-            const uint64_t tt = (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c);
+            const uint64_t tt = ntail + (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c);
             if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
         }
     }

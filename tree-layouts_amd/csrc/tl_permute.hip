@@ -278,11 +278,11 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
             const uint64_t P = op.p[0], stride = op.p[1], count = op.p[2];
             const uint32_t g = (uint32_t)std::min<uint64_t>((count + 255) / 256, 65535);
             if (op.p[3])
-                hipLaunchKernelGGL(k_copy_strided<T>, dim3(g), dim3(256), 0, s, stash, 1,
+                hipLaunchKernelGGL(k_copy_strided<T>, dim3(g), dim3(256), 0, s, stash + op.p[4], 1,
                                    (const T *)(A + P), stride, count);
             else
                 hipLaunchKernelGGL(k_copy_strided<T>, dim3(g), dim3(256), 0, s, A + P, stride,
-                                   (const T *)stash, 1, count);
+                                   (const T *)(stash + op.p[4]), 1, count);
             TL_LAUNCH_CHECK();
             break;
         }

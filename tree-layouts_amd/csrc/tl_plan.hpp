@@ -219,15 +219,17 @@ struct Planner {
         pl.ops.push_back(b2);
     }
 
-    // stash (to_stash) or unstash `count` keys at P + i*stride, i < count
-    void copy(uint64_t P, uint64_t stride, uint64_t count, bool to_stash) {
+    // stash (to_stash) or unstash `count` keys at P + i*stride, i < count,
+    // to / from stash[soff + i]
+    void copy(uint64_t P, uint64_t stride, uint64_t count, bool to_stash, uint64_t soff = 0) {
         Op op;
         op.type = OP_COPY;
         op.p[0] = P;
         op.p[1] = stride;
         op.p[2] = count;
         op.p[3] = to_stash;
-        pl.max_stash = std::max(pl.max_stash, count);
+        op.p[4] = soff;
+        pl.max_stash = std::max(pl.max_stash, soff + count);
         pl.ops.push_back(op);
     }
 
@@ -344,22 +346,31 @@ struct Planner {
             return;
         }
         for (const Run &x : runs)
-            for (uint64_t i = 0; i < x.count; i++) {
-                const uint64_t P = x.start + i * x.stride;
-                copy(P + l, l + 1, r, true);   // stash roots
-                Op op;
-                op.type = OP_CHAINED;
-                op.ms.kind = MV_SPARSE_SLIDE;
-                op.ms.P = P;
-                op.ms.r = r;
-                op.ms.l = l;
-                op.ms.CH = std::min<uint64_t>(l, 256 * 16);
-                op.ms.nchunk = (l + op.ms.CH - 1) / op.ms.CH;
-                op.p[0] = r * op.ms.nchunk;
-                pl.max_moves = std::max(pl.max_moves, op.p[0]);
-                pl.ops.push_back(op);
-                copy(P, 1, r, false);   // roots to the front
-            }
+            for (uint64_t i = 0; i < x.count; i++) sparse_slide(x.start + i * x.stride, r, l, 0, 0);
+    }
+
+    // one chained pass over r rows of (l leaf keys, 1 root) at P, followed by
+    // `tail` keys and then `extra` roots: the roots go to the stash (row roots
+    // first, then the extra roots), leaf blocks and tail slide right by `extra`
+    // more than a plain gather, and the stash comes back as the first r+extra
+    // keys.  With tail = extra = 0 this is the sparse gather.
+    void sparse_slide(uint64_t P, uint64_t r, uint64_t l, uint64_t tail, uint64_t extra) {
+        copy(P + l, l + 1, r, true);                               // row roots
+        if (extra) copy(P + r * (l + 1) + tail, 1, extra, true, r);   // trailing roots
+        Op op;
+        op.type = OP_CHAINED;
+        op.ms.kind = MV_SPARSE_SLIDE;
+        op.ms.P = P;
+        op.ms.r = r;
+        op.ms.l = l;
+        op.ms.L = tail;
+        op.ms.k = extra;
+        op.ms.CH = std::min<uint64_t>(l, 256 * 16);
+        op.ms.nchunk = (l + op.ms.CH - 1) / op.ms.CH;
+        op.p[0] = (tail + op.ms.CH - 1) / op.ms.CH + r * op.ms.nchunk;
+        pl.max_moves = std::max(pl.max_moves, op.p[0]);
+        pl.ops.push_back(op);
+        copy(P, 1, r + extra, false);   // roots to the front
     }
 };
 
@@ -477,8 +488,16 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
             for (uint64_t i = 0; i < x.count; i++) {
                 const uint64_t off = x.start + i * x.stride;
                 const uint64_t nf = (n - r) / l, inc = n - r - nf * l;
-                Q.gather_sparse({{off, 0, 1}}, nf, l);
-                if (nf < r) Q.rotate_right({{off + nf, 0, 1}}, n - nf, r - nf);
+                // the roots past the full blocks (whose bottom trees are the
+                // incomplete one and empty ones) follow the incomplete block;
+                // when the gather is a chained pass they ride along in it
+                const bool chained = nf && !(nf * Q.W <= 65536 && nf * (l + 1) <= (1ull << 22));
+                if (chained && r <= Q.tu.row_stash_max) {
+                    Q.sparse_slide(off, nf, l, inc, r - nf);
+                } else {
+                    Q.gather_sparse({{off, 0, 1}}, nf, l);
+                    if (nf < r) Q.rotate_right({{off + nf, 0, 1}}, n - nf, r - nf);
+                }
                 kids.push_back({r, {{off, 0, 1}}});
                 if (nf) kids.push_back({l, {{off + r, l, nf}}});
                 if (inc) kids.push_back({inc, {{off + n - inc, 0, 1}}});
@@ -543,12 +562,17 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             break;
         }
         case OP_SPARSE_STREAM: end = runs_end(pl, op, op.p[0] * (op.p[1] + 1)); break;
-        case OP_COPY: end = op.p[0] + (op.p[2] ? (op.p[2] - 1) * op.p[1] + 1 : 0); break;
+        case OP_COPY:
+            end = op.p[0] + (op.p[2] ? (op.p[2] - 1) * op.p[1] + 1 : 0);
+            if (op.p[4] + op.p[2] > pl.max_stash) neg = true;
+            break;
         case OP_CHAINED: {
             const MoveSpec &m = op.ms;
             if (m.kind == MV_SPARSE_SLIDE) {
-                end = m.P + m.r * (m.l + 1);
-                if (m.CH == 0 || m.CH > 256 * 16 || m.CH > m.l) neg = true;
+                end = m.P + m.r * (m.l + 1) + m.L + m.k;
+                if (m.CH == 0 || m.CH > 256 * 16 || m.CH > m.l ||
+                    op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
+                    neg = true;
             } else if (m.kind == MV_ROW_SLIDE) {
                 end = m.P + m.r * (m.l + 1);
                 if (m.CH == 0 || m.CH * (m.l + 1) > 256 * 16 || m.nchunk * m.CH < m.r ||
