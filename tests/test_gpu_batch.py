@@ -1,8 +1,9 @@
 """Uneven batch: every array of a mixed batch (sizes spread over four orders
 of magnitude, int32/uint64 keys, BST / B-tree b=8 / mvEB) comes out of the
-batch engine with the oracle's layout and the oracle's answers.  The per-array
-path runs; the single persistent launch is disabled (its tests are skipped
-with the reason) and must fail loudly."""
+batch engine with the oracle's layout and the oracle's answers: the planned
+path (tl_batch_run, eager and replayed from its HIP graph) and the per-array
+calls (tl_batch_run_separate).  The single persistent launch is disabled; its
+tests are skipped with the reason."""
 import numpy as np
 import pytest
 import torch
@@ -90,12 +91,34 @@ def test_batch_separate_path_matches(dev):
     B.close()
 
 
-def test_persistent_launch_fails_loudly(dev):
-    arrays, host = make_batch(6, 2, 8, 8, dev)
-    B = tl_amd.Batch(arrays)
-    with pytest.raises(tl_amd.TLError) as e:
-        B.run(graph=False)
-    assert e.value.status == tl_amd.TL_EUNSUPPORTED
+@pytest.mark.parametrize("lds", [0, 32768])
+def test_batch_planned_path_matches_oracle(lds, dev):
+    """tl_batch_run (the planned per-array path: small arrays in batched LDS
+    launches, the others on their own plans, queries per array) matches the
+    oracle eagerly and on three replays of its HIP graph, with the arrays
+    restored and the program's own work and synchronizations in between."""
+    arrays, host = make_batch(240, 0, 18, 5, dev)
+    B = tl_amd.Batch(arrays, lds_bytes=lds)
+    B.run(graph=False)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    for _ in range(3):
+        restore(arrays, host, dev)
+        torch.cuda.synchronize()
+        B.run(graph=True)
+        torch.cuda.synchronize()
+        check(arrays, host)
+    B.close()
+
+
+def test_batch_planned_path_autotune(dev):
+    arrays, host = make_batch(90, 6, 16, 7, dev)
+    cfg = tl_amd.batch_autotune(arrays, sample=30)
+    assert cfg["lds_bytes"] in (32768, 65536, 98304) and cfg["wg_per_cu"] in (2, 4, 8)
+    B = tl_amd.Batch(arrays, lds_bytes=cfg["lds_bytes"], wg_per_cu=cfg["wg_per_cu"])
+    B.run(graph=True)
+    torch.cuda.synchronize()
+    check(arrays, host)
     B.close()
 
 

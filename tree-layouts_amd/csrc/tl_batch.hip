@@ -17,6 +17,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <memory>
 #include <vector>
 
@@ -487,8 +489,35 @@ __global__ void k_batch_clear(unsigned long long *done, uint64_t nops, uint32_t 
 // ---------------------------------------------------------------------------
 // host side
 
+// ---- the planned per-array path.  Every array's plan is built and uploaded
+// when the batch is created.  Arrays whose whole layout fits in LDS share
+// batched launches (one workgroup per array, grouped by layout kind, b, key
+// width and size class); every other array runs its own plan's launches on
+// batch-owned stash and flags.  A run does no host work, copies or memsets, so
+// it is captured once in a HIP graph and replayed.
+struct SmallGroup {
+    uint32_t W, kind, b, max_n;
+    std::vector<uint64_t> desc;   // (address, n) pairs
+    uint64_t at = 0;              // byte offset of desc in the planned data
+};
+struct LargeItem {
+    uint32_t item, W;
+    Plan plan;
+    uint64_t at = 0;              // byte offset of the plan blob
+};
+struct QTables {                  // device mvEB table set of one array
+    uint64_t tables_at = 0, idx_at = 0, tidx_at = 0;
+    uint32_t nt = 0;
+};
+
 struct Batch {
     std::vector<tl_batch_item> items;
+    std::vector<SmallGroup> small;
+    std::vector<LargeItem> large;
+    std::vector<QTables> qt;      // per item (layout 2 with queries)
+    void *d_pdata = nullptr, *d_pstash = nullptr;
+    uint32_t *d_pflags = nullptr;
+    uint64_t small_items = 0;
     tl_batch_config cfg{};
     std::vector<BOp> ops;
     std::vector<BTask> tasks;
@@ -510,7 +539,8 @@ struct Batch {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : {(void *)d_ops, (void *)d_tasks, (void *)d_blob, (void *)d_done,
-                        (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash})
+                        (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash, d_pdata,
+                        d_pstash, (void *)d_pflags})
             if (p) (void)hipFree(p);
     }
 };
@@ -717,6 +747,114 @@ static tl_status upload_batch(Batch &B) {
     return TL_OK;
 }
 
+static tl_status build_planned(Batch &B, Ctx *c) {
+    Tuning tu = tuning_for(c);
+    if (B.cfg.lds_bytes)
+        tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
+    std::vector<unsigned char> data;
+    auto put = [&data](const void *src, size_t bytes) -> uint64_t {
+        const uint64_t at = (data.size() + 63) & ~63ull;
+        data.resize(at + bytes);
+        if (bytes) memcpy(data.data() + at, src, bytes);
+        return at;
+    };
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, size_t> group_of;
+    size_t stash_bytes = 64, flag_bytes = 64;
+    B.qt.assign(B.items.size(), QTables{});
+    for (uint32_t a = 0; a < B.items.size(); a++) {
+        const tl_batch_item &it = B.items[a];
+        uint32_t W;
+        width_of(it.dtype, W);
+        const uint64_t b = it.layout == 0 ? 1 : it.b;
+        Plan pl;
+        build_plan(pl, tu, (Layout)it.layout, it.n, b, W);
+        std::string why;
+        if (!validate_plan(pl, it.n, why)) {
+            set_error(why);
+            return TL_EINVAL;
+        }
+        if (pl.ops.size() == 1 && pl.ops[0].type == OP_LDS && pl.ops[0].nitems == 1 &&
+            pl.ops[0].p[0] == it.n) {
+            // the whole array in LDS: size classes of 4, 16, 64 KiB and the budget
+            const uint64_t bytes = it.n * W;
+            const uint32_t cls = bytes <= 4096 ? 0 : bytes <= 16384 ? 1 : bytes <= 65536 ? 2 : 3;
+            const auto key = std::make_tuple(W, (uint32_t)pl.ops[0].p[1], (uint32_t)pl.ops[0].p[2], cls);
+            auto f = group_of.find(key);
+            if (f == group_of.end()) {
+                f = group_of.emplace(key, B.small.size()).first;
+                SmallGroup g;
+                g.W = W;
+                g.kind = (uint32_t)pl.ops[0].p[1];
+                g.b = (uint32_t)pl.ops[0].p[2];
+                g.max_n = 0;
+                B.small.push_back(g);
+            }
+            SmallGroup &g = B.small[f->second];
+            g.desc.push_back((uint64_t)it.A);
+            g.desc.push_back(it.n);
+            g.max_n = std::max<uint32_t>(g.max_n, (uint32_t)it.n);
+            B.small_items++;
+        } else if (!pl.ops.empty()) {
+            LargeItem L;
+            L.item = a;
+            L.W = W;
+            L.at = put(pl.blob.data(), pl.blob.size() * 8);
+            stash_bytes = std::max<size_t>(stash_bytes, pl.max_stash * W);
+            if (pl.max_moves) flag_bytes = std::max(flag_bytes, plan_flag_bytes(pl));
+            L.plan = std::move(pl);
+            B.large.push_back(std::move(L));
+        }
+        if (it.layout == 2 && it.nq && it.n) {
+            std::vector<uint64_t> tb, ix, tix;
+            host_tableset(it.n, tb, ix, tix);
+            std::vector<tl_veb_table> t(tb.size() / 3);
+            for (size_t k = 0; k < t.size(); k++)
+                t[k] = tl_veb_table{tb[3 * k], tb[3 * k + 1], (uint32_t)tb[3 * k + 2]};
+            std::vector<uint32_t> t32(tix.begin(), tix.end());
+            QTables &q = B.qt[a];
+            q.tables_at = put(t.data(), t.size() * sizeof(tl_veb_table));
+            q.idx_at = put(ix.data(), ix.size() * 8);
+            q.tidx_at = put(t32.data(), t32.size() * 4);
+            q.nt = (uint32_t)ix.size();
+        }
+    }
+    for (SmallGroup &g : B.small) g.at = put(g.desc.data(), g.desc.size() * 8);
+    TL_HIP(hipMalloc(&B.d_pdata, std::max<size_t>(data.size(), 64)));
+    if (!data.empty())
+        TL_HIP(hipMemcpy(B.d_pdata, data.data(), data.size(), hipMemcpyHostToDevice));
+    TL_HIP(hipMalloc(&B.d_pstash, stash_bytes));
+    TL_HIP(hipMalloc((void **)&B.d_pflags, flag_bytes));
+    return TL_OK;
+}
+
+static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
+    const unsigned char *base = (const unsigned char *)B.d_pdata;
+    TL_TRY(fill_u32_launch(c->err_flag, 1, 0u, s));
+    for (const SmallGroup &g : B.small)
+        TL_TRY(lds_items_launch((const uint64_t *)(base + g.at), (uint32_t)(g.desc.size() / 2), g.W,
+                                g.kind, g.b, g.max_n, s));
+    for (const LargeItem &L : B.large)
+        TL_TRY(plan_launch(c, L.plan, B.items[L.item].A, L.W, s,
+                           (const uint64_t *)(base + L.at), B.d_pstash, B.d_pflags));
+    for (uint32_t a = 0; a < B.items.size(); a++) {
+        const tl_batch_item &it = B.items[a];
+        if (!it.nq) continue;
+        if (it.layout == 0) {
+            TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+        } else if (it.layout == 1) {
+            TL_TRY(tl_query_btree(it.A, it.n, it.b, it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+        } else {
+            const QTables &q = B.qt[a];
+            const uint32_t d = floor_log2_64(it.n ? it.n : 1) + 1;
+            TL_TRY(tl_query_mveb_nonperfect(it.A, (const tl_veb_table *)(base + q.tables_at),
+                                            (const uint64_t *)(base + q.idx_at),
+                                            (const uint32_t *)(base + q.tidx_at), it.n, d, q.nt,
+                                            it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+        }
+    }
+    return TL_OK;
+}
+
 static void launch_batch(Batch &B, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_clear, dim3(256), dim3(256), 0, s, B.d_done, (uint64_t)B.ops.size(),
                        B.d_flags, B.nflags, B.d_ticket, B.d_err);
@@ -744,6 +882,7 @@ extern "C" tl_status tl_batch_create(const tl_batch_item *items, uint32_t count,
     if (B->cfg.wg_per_cu == 0) B->cfg.wg_per_cu = 4;
     TL_TRY(build_batch(*B, c));
     TL_TRY(upload_batch(*B));
+    TL_TRY(build_planned(*B, c));
     *handle = B.release();
     return TL_OK;
 }
@@ -763,9 +902,19 @@ extern "C" tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks
 // count and again with relaxed polling bounded by wall-clock time; the host
 // replay of the same sub-task list is correct and every wait targets a
 // smaller ticket, so the stall is not a scheduling deadlock of the task list.
-// Until its cause is shown the launch stays off; the sub-task list is still
-// built, exported and replayed on the host.
+// Until its cause is shown the launch stays off and tl_batch_run runs the
+// planned per-array path; the sub-task list is still built, exported and
+// replayed on the host.
 static bool persistent_launch_enabled() { return false; }
+
+static tl_status enqueue_run(Batch &B, Ctx *c, hipStream_t s) {
+    if (persistent_launch_enabled()) {
+        launch_batch(B, s);
+        TL_LAUNCH_CHECK();
+        return TL_OK;
+    }
+    return run_planned(B, c, s);
+}
 
 extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms) {
     if (!handle) {
@@ -773,11 +922,8 @@ extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, flo
         return TL_EINVAL;
     }
     Batch *B = (Batch *)handle;
-    if (!persistent_launch_enabled()) {
-        set_error("tl_batch_run: the persistent launch is disabled: it stalled on gfx950 and the "
-                  "cause is not identified; tl_batch_run_separate runs the same work");
-        return TL_EUNSUPPORTED;
-    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
     hipStream_t s = (hipStream_t)stream;
     Timer tm;
     if (use_graph) {
@@ -789,9 +935,15 @@ extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, flo
             hipStream_t cap;
             TL_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             TL_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-            launch_batch(*B, cap);
-            TL_HIP(hipStreamEndCapture(cap, &B->graph));
-            TL_HIP(hipStreamDestroy(cap));
+            const tl_status st = enqueue_run(*B, c, cap);
+            const hipError_t ec = hipStreamEndCapture(cap, &B->graph);
+            (void)hipStreamDestroy(cap);
+            if (st != TL_OK) {
+                if (B->graph) (void)hipGraphDestroy(B->graph);
+                B->graph = nullptr;
+                return st;
+            }
+            TL_HIP(ec);
             TL_HIP(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
             B->graph_stream = s;
         }
@@ -799,16 +951,19 @@ extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, flo
         TL_HIP(hipGraphLaunch(B->gexec, s));
     } else {
         TL_TRY(tm.start(s, ms));
-        launch_batch(*B, s);
-        TL_LAUNCH_CHECK();
+        TL_TRY(enqueue_run(*B, c, s));
     }
     TL_TRY(tm.stop());
     if (ms) {
-        uint32_t e = 0;
-        TL_HIP(hipMemcpy(&e, B->d_err, 4, hipMemcpyDeviceToHost));
-        if (e) {
-            set_error("tl_batch_run: an in-kernel bounded wait timed out");
-            return TL_ETIMEOUT;
+        if (persistent_launch_enabled()) {
+            uint32_t e = 0;
+            TL_HIP(hipMemcpy(&e, B->d_err, 4, hipMemcpyDeviceToHost));
+            if (e) {
+                set_error("tl_batch_run: an in-kernel bounded wait timed out");
+                return TL_ETIMEOUT;
+            }
+        } else {
+            TL_TRY(check_err_flag(c, s));
         }
     }
     return TL_OK;

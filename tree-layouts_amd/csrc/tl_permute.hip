@@ -31,6 +31,13 @@
 
 namespace tl {
 
+// graph-safe clear (no memset nodes in captured work)
+__global__ void k_fill_u32(uint32_t *p, uint64_t count, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // in-LDS layout over runs of items, `per` items per workgroup.  With several
 // items per workgroup the slot of every in-item index is computed once into
 // an LDS table behind the keys; i / n uses a multiply by ceil(2^32 / n),
@@ -207,14 +214,8 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
 // executor
 
 template <typename T>
-static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
-    uint64_t *blob = nullptr;
-    if (!pl.blob.empty())
-        TL_TRY(copy_to_device(c, 2, pl.blob.data(), pl.blob.size() * 8, s, (void **)&blob));
-    T *stash = nullptr;
-    if (pl.max_stash) TL_TRY(scratch(c, 0, pl.max_stash * sizeof(T), (void **)&stash));
-    uint32_t *flags = nullptr;
-    if (pl.max_moves) TL_TRY(scratch(c, 1, (((pl.max_moves + 15) / 16) * 16 + 16) * 4, (void **)&flags));
+static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const uint64_t *blob,
+                             T *stash, uint32_t *flags, bool graph_safe) {
     for (const Op &op : pl.ops) {
         const uint64_t *runs = blob ? blob + op.runs_at : nullptr;
         switch (op.type) {
@@ -289,7 +290,12 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
         case OP_CHAINED: {
             const uint64_t nmoves = op.p[0];
             const uint64_t words = ((nmoves + 15) / 16) * 16;
-            TL_HIP(hipMemsetAsync(flags, 0, (words + 16) * 4, s));
+            if (graph_safe) {
+                hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(256), 0, s, flags, words + 16, 0u);
+                TL_LAUNCH_CHECK();
+            } else {
+                TL_HIP(hipMemsetAsync(flags, 0, (words + 16) * 4, s));
+            }
             unsigned long long *counter = (unsigned long long *)(flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
             hipLaunchKernelGGL((k_chained<T, 16>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
@@ -299,6 +305,76 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
         }
         }
     }
+    return TL_OK;
+}
+
+template <typename T>
+static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
+    uint64_t *blob = nullptr;
+    if (!pl.blob.empty())
+        TL_TRY(copy_to_device(c, 2, pl.blob.data(), pl.blob.size() * 8, s, (void **)&blob));
+    T *stash = nullptr;
+    if (pl.max_stash) TL_TRY(scratch(c, 0, pl.max_stash * sizeof(T), (void **)&stash));
+    uint32_t *flags = nullptr;
+    if (pl.max_moves) TL_TRY(scratch(c, 1, plan_flag_bytes(pl), (void **)&flags));
+    return launch_plan<T>(c, pl, A, s, blob, stash, flags, false);
+}
+
+tl_status fill_u32_launch(uint32_t *p, uint64_t count, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(256), 0, s, p, count, v);
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
+tl_status plan_launch(Ctx *c, const Plan &pl, void *A, uint32_t W, hipStream_t s,
+                      const uint64_t *dev_blob, void *stash, uint32_t *flags) {
+    return W == 4 ? launch_plan<uint32_t>(c, pl, (uint32_t *)A, s, dev_blob, (uint32_t *)stash,
+                                          flags, true)
+                  : launch_plan<uint64_t>(c, pl, (uint64_t *)A, s, dev_blob, (uint64_t *)stash,
+                                          flags, true);
+}
+
+// whole small arrays in LDS, one workgroup each: desc holds (address, n)
+// pairs of arrays of one layout kind, b and key width, n <= max_n
+template <typename T>
+__global__ __launch_bounds__(512) void k_lds_items(const uint64_t *__restrict__ desc, uint32_t kind,
+                                                   uint32_t b) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *s = reinterpret_cast<T *>(smem_raw);
+    T *A = reinterpret_cast<T *>(desc[2 * blockIdx.x]);
+    const uint32_t n = (uint32_t)desc[2 * blockIdx.x + 1];
+    BtreeShape sh;
+    if (kind == LDS_BTREE) sh = btree_shape(n, b);
+    constexpr int U = 8;
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += blockDim.x * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * blockDim.x;
+            if (i < n) v[u] = A[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * blockDim.x;
+            if (i < n) s[lds_slot(i, n, kind, b, 0, sh)] = v[u];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) A[i] = s[i];
+}
+
+tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W, uint32_t kind,
+                           uint32_t b, uint32_t max_n, hipStream_t s) {
+    if (!count) return TL_OK;
+    const size_t bytes = (size_t)max_n * W;
+    const int threads = max_n >= 4096 ? 512 : 256;
+    if (W == 4)
+        hipLaunchKernelGGL(k_lds_items<uint32_t>, dim3(count), dim3(threads), bytes, s, dev_desc,
+                           kind, b);
+    else
+        hipLaunchKernelGGL(k_lds_items<uint64_t>, dim3(count), dim3(threads), bytes, s, dev_desc,
+                           kind, b);
+    TL_LAUNCH_CHECK();
     return TL_OK;
 }
 

@@ -75,6 +75,21 @@ struct Plan {
     }
 };
 
+// bytes of chained-move flags (and the ticket behind them) a plan needs
+inline size_t plan_flag_bytes(const Plan &pl) {
+    return (((pl.max_moves + 15) / 16) * 16 + 16) * 4;
+}
+
+// launch a plan whose blob is already on the device, with caller-owned stash
+// and flags, clearing flags by a kernel (so the launches can be captured)
+tl_status plan_launch(Ctx *c, const Plan &pl, void *A, uint32_t W, hipStream_t s,
+                      const uint64_t *dev_blob, void *stash, uint32_t *flags);
+// p[0, count) = v by a kernel
+tl_status fill_u32_launch(uint32_t *p, uint64_t count, uint32_t v, hipStream_t s);
+// whole arrays of at most max_n keys in LDS, one workgroup per (address, n)
+tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W, uint32_t kind,
+                           uint32_t b, uint32_t max_n, hipStream_t s);
+
 // LDS kept free for kernels' static shared variables next to the dynamic area
 constexpr size_t LDS_STATIC_RESERVE = 2048;
 
