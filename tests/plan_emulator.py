@@ -9,7 +9,8 @@ import numpy as np
 
 import oracle_lib as ol
 
-OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED = range(7)
+(OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
+ OP_BITREV_LEVELS) = range(8)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV = range(4)
 MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE = range(4)
 
@@ -99,6 +100,13 @@ def replay(buf, A):
             ok = idx >= 0
             old = A.copy()
             A[idx[ok]] = old[par[ok]]
+        elif t == OP_BITREV_LEVELS:
+            for lev in range(p[0], p[1]):
+                y = np.arange(1 << lev, dtype=np.int64)
+                idx = np.int64((1 << lev) - 1) + y
+                par = np.int64((1 << lev) - 1) + rev_bits(y, lev)
+                old = A.copy()
+                A[idx] = old[par]
         elif t == OP_UNIT_MERGE:
             unit, cs_at, ncyc, cu_at = p[0], p[1], p[2], p[3]
             cs = blob[cs_at: cs_at + ncyc + 1]

@@ -504,6 +504,7 @@ struct LargeItem {
     uint32_t item, W;
     Plan plan;
     uint64_t at = 0;              // byte offset of the plan blob
+    uint64_t flags_at = 0;        // byte offset of its chained-move flags
 };
 struct QTables {                  // device mvEB table set of one array
     uint64_t tables_at = 0, idx_at = 0, tidx_at = 0;
@@ -518,6 +519,12 @@ struct Batch {
     void *d_pdata = nullptr, *d_pstash = nullptr;
     uint32_t *d_pflags = nullptr;
     uint64_t small_items = 0;
+    // concurrent branches: each has its own stash and flags
+    static constexpr int NSUB = 4;
+    hipStream_t sub[NSUB] = {};
+    hipEvent_t ev_fork[2] = {}, ev_join[2][NSUB] = {};
+    size_t stash_stride = 0, flag_stride = 0;
+    std::vector<int> large_branch, query_branch, small_branch;
     tl_batch_config cfg{};
     std::vector<BOp> ops;
     std::vector<BTask> tasks;
@@ -542,6 +549,13 @@ struct Batch {
                         (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash, d_pdata,
                         d_pstash, (void *)d_pflags})
             if (p) (void)hipFree(p);
+        for (int k = 0; k < NSUB; k++)
+            if (sub[k]) (void)hipStreamDestroy(sub[k]);
+        for (int f = 0; f < 2; f++) {
+            if (ev_fork[f]) (void)hipEventDestroy(ev_fork[f]);
+            for (int k = 0; k < NSUB; k++)
+                if (ev_join[f][k]) (void)hipEventDestroy(ev_join[f][k]);
+        }
     }
 };
 
@@ -638,7 +652,22 @@ static tl_status build_batch(Batch &B, Ctx *c) {
         stash_elems += pl.max_stash;
         int64_t prev = -1;
         uint32_t depth = 0;
+        // the persistent executors know single-level reversals only
+        std::vector<Op> flat;
         for (const Op &op : pl.ops) {
+            if (op.type != OP_BITREV_LEVELS) {
+                flat.push_back(op);
+                continue;
+            }
+            for (uint64_t t = op.p[0]; t < op.p[1]; t++) {
+                Op one = op;
+                one.type = OP_BITREV;
+                one.p[0] = (1ull << t) - 1;
+                one.p[1] = t;
+                flat.push_back(one);
+            }
+        }
+        for (const Op &op : flat) {
             BOp bo{};
             bo.type = op.type;
             bo.width = W;
@@ -759,7 +788,7 @@ static tl_status build_planned(Batch &B, Ctx *c) {
         return at;
     };
     std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, size_t> group_of;
-    size_t stash_bytes = 64, flag_bytes = 64;
+    size_t stash_bytes = 64, flag_bytes = 256;
     B.qt.assign(B.items.size(), QTables{});
     for (uint32_t a = 0; a < B.items.size(); a++) {
         const tl_batch_item &it = B.items[a];
@@ -800,7 +829,8 @@ static tl_status build_planned(Batch &B, Ctx *c) {
             L.W = W;
             L.at = put(pl.blob.data(), pl.blob.size() * 8);
             stash_bytes = std::max<size_t>(stash_bytes, pl.max_stash * W);
-            if (pl.max_moves) flag_bytes = std::max(flag_bytes, plan_flag_bytes(pl));
+            L.flags_at = flag_bytes;
+            flag_bytes += (plan_flag_bytes_all(pl) + 255) & ~255ull;
             L.plan = std::move(pl);
             B.large.push_back(std::move(L));
         }
@@ -822,37 +852,99 @@ static tl_status build_planned(Batch &B, Ctx *c) {
     TL_HIP(hipMalloc(&B.d_pdata, std::max<size_t>(data.size(), 64)));
     if (!data.empty())
         TL_HIP(hipMemcpy(B.d_pdata, data.data(), data.size(), hipMemcpyHostToDevice));
-    TL_HIP(hipMalloc(&B.d_pstash, stash_bytes));
+    B.stash_stride = (stash_bytes + 255) & ~255ull;
+    B.flag_stride = flag_bytes;   // one region for all items: cleared once per run
+    TL_HIP(hipMalloc(&B.d_pstash, B.stash_stride * Batch::NSUB));
     TL_HIP(hipMalloc((void **)&B.d_pflags, flag_bytes));
+    for (int k = 0; k < Batch::NSUB; k++)
+        TL_HIP(hipStreamCreateWithFlags(&B.sub[k], hipStreamNonBlocking));
+    for (int f = 0; f < 2; f++) {
+        TL_HIP(hipEventCreateWithFlags(&B.ev_fork[f], hipEventDisableTiming));
+        for (int k = 0; k < Batch::NSUB; k++)
+            TL_HIP(hipEventCreateWithFlags(&B.ev_join[f][k], hipEventDisableTiming));
+    }
+    // longest work first onto the least loaded branch
+    auto lpt = [](const std::vector<double> &cost, std::vector<int> &branch) {
+        std::vector<size_t> order(cost.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&cost](size_t x, size_t y) { return cost[x] > cost[y]; });
+        double load[Batch::NSUB] = {0, 0, 0, 0};
+        branch.assign(cost.size(), 0);
+        for (size_t i : order) {
+            int k = 0;
+            for (int j = 1; j < Batch::NSUB; j++)
+                if (load[j] < load[k]) k = j;
+            branch[i] = k;
+            load[k] += cost[i] + 1.0;
+        }
+    };
+    std::vector<double> lc, qc, sc;
+    for (const LargeItem &L : B.large)
+        lc.push_back((double)B.items[L.item].n * L.W * (double)L.plan.ops.size());
+    for (const tl_batch_item &it : B.items) qc.push_back((double)it.nq);
+    for (const SmallGroup &g : B.small) sc.push_back((double)g.desc.size() * g.max_n);
+    lpt(lc, B.large_branch);
+    lpt(qc, B.query_branch);
+    lpt(sc, B.small_branch);
+    return TL_OK;
+}
+
+// the branches start after everything enqueued on s and s continues after all
+// of them (events; under capture this makes the graph's parallel branches)
+static tl_status fork_branches(Batch &B, int f, hipStream_t s) {
+    TL_HIP(hipEventRecord(B.ev_fork[f], s));
+    for (int k = 0; k < Batch::NSUB; k++) TL_HIP(hipStreamWaitEvent(B.sub[k], B.ev_fork[f], 0));
+    return TL_OK;
+}
+static tl_status join_branches(Batch &B, int f, hipStream_t s) {
+    for (int k = 0; k < Batch::NSUB; k++) {
+        TL_HIP(hipEventRecord(B.ev_join[f][k], B.sub[k]));
+        TL_HIP(hipStreamWaitEvent(s, B.ev_join[f][k], 0));
+    }
     return TL_OK;
 }
 
 static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
     const unsigned char *base = (const unsigned char *)B.d_pdata;
     TL_TRY(fill_u32_launch(c->err_flag, 1, 0u, s));
-    for (const SmallGroup &g : B.small)
-        TL_TRY(lds_items_launch((const uint64_t *)(base + g.at), (uint32_t)(g.desc.size() / 2), g.W,
-                                g.kind, g.b, g.max_n, s));
-    for (const LargeItem &L : B.large)
-        TL_TRY(plan_launch(c, L.plan, B.items[L.item].A, L.W, s,
-                           (const uint64_t *)(base + L.at), B.d_pstash, B.d_pflags));
+    TL_TRY(fill_u32_launch(B.d_pflags, B.flag_stride / 4, 0u, s));
+    // permutations: every array on one branch, with that branch's stash and flags
+    TL_TRY(fork_branches(B, 0, s));
+    for (size_t g = 0; g < B.small.size(); g++) {
+        const SmallGroup &G = B.small[g];
+        TL_TRY(lds_items_launch((const uint64_t *)(base + G.at), (uint32_t)(G.desc.size() / 2), G.W,
+                                G.kind, G.b, G.max_n, B.sub[B.small_branch[g]]));
+    }
+    for (size_t i = 0; i < B.large.size(); i++) {
+        const LargeItem &L = B.large[i];
+        const int k = B.large_branch[i];
+        TL_TRY(plan_launch(c, L.plan, B.items[L.item].A, L.W, B.sub[k],
+                           (const uint64_t *)(base + L.at),
+                           (unsigned char *)B.d_pstash + k * B.stash_stride,
+                           (uint32_t *)((unsigned char *)B.d_pflags + L.flags_at)));
+    }
+    TL_TRY(join_branches(B, 0, s));
+    // queries, once every layout is in place
+    TL_TRY(fork_branches(B, 1, s));
     for (uint32_t a = 0; a < B.items.size(); a++) {
         const tl_batch_item &it = B.items[a];
         if (!it.nq) continue;
+        hipStream_t q_s = B.sub[B.query_branch[a]];
         if (it.layout == 0) {
-            TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+            TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, q_s, nullptr));
         } else if (it.layout == 1) {
-            TL_TRY(tl_query_btree(it.A, it.n, it.b, it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+            TL_TRY(tl_query_btree(it.A, it.n, it.b, it.dtype, it.Q, it.nq, it.ans, q_s, nullptr));
         } else {
             const QTables &q = B.qt[a];
             const uint32_t d = floor_log2_64(it.n ? it.n : 1) + 1;
             TL_TRY(tl_query_mveb_nonperfect(it.A, (const tl_veb_table *)(base + q.tables_at),
                                             (const uint64_t *)(base + q.idx_at),
                                             (const uint32_t *)(base + q.tidx_at), it.n, d, q.nt,
-                                            it.dtype, it.Q, it.nq, it.ans, s, nullptr));
+                                            it.dtype, it.Q, it.nq, it.ans, q_s, nullptr));
         }
     }
-    return TL_OK;
+    return join_branches(B, 1, s);
 }
 
 static void launch_batch(Batch &B, hipStream_t s) {

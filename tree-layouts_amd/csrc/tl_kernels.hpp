@@ -38,16 +38,14 @@ __device__ inline uint64_t bitrev_tile_order(uint64_t t, uint32_t MB, uint32_t p
     return (Ahi << (sb + c)) | (C << sb) | B;
 }
 
+// one tile pair of a D-bit reversal at base: tile number t of 2^(D-2K)
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
-                                                      uint32_t D) {
+__device__ inline void bitrev_tile(T *__restrict__ A, int64_t base, uint32_t D, uint64_t t) {
     constexpr int R = 1 << K;
     constexpr int E = R * R / 256;   // elements per thread and tile
     __shared__ T sA[R][R + 1];
     __shared__ T sB[R][R + 1];
     const uint32_t MB = D - 2 * K;
-    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
-    if (MB < 64 && (t >> MB) != 0) return;
     constexpr uint32_t row_log2 = (uint32_t)K + (sizeof(T) == 8 ? 3u : 2u);
     const uint64_t m = bitrev_tile_order(t, MB, 21u - row_log2);
     const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
@@ -77,6 +75,27 @@ __global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t
             A[ia] = sA[hr][lr];
         }
     }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
+                                                      uint32_t D) {
+    const uint32_t MB = D - 2 * K;
+    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (MB < 64 && (t >> MB) != 0) return;
+    bitrev_tile<T, K>(A, base, D, t);
+}
+
+// the per-level reversals of levels t0 .. d-1 (t0 = 2K) in one launch: level t
+// holds 2^t keys at base 2^t - 1 and 2^(t-2K) tiles; tile number g of the
+// launch is tile g - (2^j - 1) of level t0 + j, j = floor(log2(g + 1))
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_bitrev_levels(T *__restrict__ A, uint32_t t0, uint32_t d) {
+    const uint64_t g = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (g >= (1ull << (d - t0)) - 1) return;
+    const uint32_t j = 63u - (uint32_t)__clzll(g + 1);
+    const uint32_t t = t0 + j;
+    bitrev_tile<T, K>(A, (int64_t)((1ull << t) - 1), t, g - ((1ull << j) - 1));
 }
 
 // ---------------------------------------------------------------------------

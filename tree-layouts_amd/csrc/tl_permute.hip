@@ -215,7 +215,10 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
 
 template <typename T>
 static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const uint64_t *blob,
-                             T *stash, uint32_t *flags, bool graph_safe) {
+                             T *stash, uint32_t *flags, bool preclear) {
+    // preclear: every chained op gets its own flag region (plan_flag_bytes_all),
+    // zeroed by the caller beforehand; otherwise one region, cleared per op
+    uint32_t *op_flags = flags;
     for (const Op &op : pl.ops) {
         const uint64_t *runs = blob ? blob + op.runs_at : nullptr;
         switch (op.type) {
@@ -257,6 +260,18 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             TL_LAUNCH_CHECK();
             break;
         }
+        case OP_BITREV_LEVELS: {
+            const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
+            const uint64_t tiles = (1ull << (d - t0)) - 1;
+            const uint32_t gx = (uint32_t)std::min<uint64_t>(tiles, 1u << 20);
+            const uint32_t gy = (uint32_t)((tiles + gx - 1) / gx);
+            if (K == 6)
+                hipLaunchKernelGGL((k_bitrev_levels<T, 6>), dim3(gx, gy), dim3(256), 0, s, A, t0, d);
+            else
+                hipLaunchKernelGGL((k_bitrev_levels<T, 5>), dim3(gx, gy), dim3(256), 0, s, A, t0, d);
+            TL_LAUNCH_CHECK();
+            break;
+        }
         case OP_UNIT_MERGE: {
             const uint64_t unit = op.p[0];
             const uint32_t nchunks = (uint32_t)((unit + 255) / 256);
@@ -290,17 +305,13 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
         case OP_CHAINED: {
             const uint64_t nmoves = op.p[0];
             const uint64_t words = ((nmoves + 15) / 16) * 16;
-            if (graph_safe) {
-                hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(256), 0, s, flags, words + 16, 0u);
-                TL_LAUNCH_CHECK();
-            } else {
-                TL_HIP(hipMemsetAsync(flags, 0, (words + 16) * 4, s));
-            }
-            unsigned long long *counter = (unsigned long long *)(flags + words);
+            if (!preclear) TL_HIP(hipMemsetAsync(op_flags, 0, (words + 16) * 4, s));
+            unsigned long long *counter = (unsigned long long *)(op_flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
             hipLaunchKernelGGL((k_chained<T, 16>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
-                               flags, counter, c->err_flag, c->wait_ticks, stash);
+                               op_flags, counter, c->err_flag, c->wait_ticks, stash);
             TL_LAUNCH_CHECK();
+            if (preclear) op_flags += words + 16;
             break;
         }
         }
@@ -321,7 +332,8 @@ static tl_status run_plan(Ctx *c, Plan &pl, T *A, hipStream_t s) {
 }
 
 tl_status fill_u32_launch(uint32_t *p, uint64_t count, uint32_t v, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(256), 0, s, p, count, v);
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((count + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, p, count, v);
     TL_LAUNCH_CHECK();
     return TL_OK;
 }

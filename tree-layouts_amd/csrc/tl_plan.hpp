@@ -40,7 +40,16 @@ __device__ inline uint64_t run_offset(const uint64_t *runs, uint32_t nruns, uint
 // ---------------------------------------------------------------------------
 // plan
 
-enum OpType { OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED };
+enum OpType {
+    OP_LDS,
+    OP_REVERSE,
+    OP_BITREV,
+    OP_UNIT_MERGE,
+    OP_SPARSE_STREAM,
+    OP_COPY,
+    OP_CHAINED,
+    OP_BITREV_LEVELS   // the per-level reversals of levels p[0] .. p[1]-1, tiles 2^p[2]
+};
 
 struct Op {
     OpType type;
@@ -80,8 +89,17 @@ inline size_t plan_flag_bytes(const Plan &pl) {
     return (((pl.max_moves + 15) / 16) * 16 + 16) * 4;
 }
 
+// bytes of flags when every chained op of the plan has a region of its own
+inline size_t plan_flag_bytes_all(const Plan &pl) {
+    size_t words = 0;
+    for (const Op &op : pl.ops)
+        if (op.type == OP_CHAINED) words += ((op.p[0] + 15) / 16) * 16 + 16;
+    return words * 4;
+}
+
 // launch a plan whose blob is already on the device, with caller-owned stash
-// and flags, clearing flags by a kernel (so the launches can be captured)
+// and flags (plan_flag_bytes_all bytes, zeroed by the caller), so that the
+// launches can be captured
 tl_status plan_launch(Ctx *c, const Plan &pl, void *A, uint32_t W, hipStream_t s,
                       const uint64_t *dev_blob, void *stash, uint32_t *flags);
 // p[0, count) = v by a kernel
@@ -351,7 +369,9 @@ struct Planner {
         for (auto &x : runs) items += x.count;
         const uint64_t part = r * (l + 1);
         const bool roots_fit = r * W <= 65536;
-        if (roots_fit && (items >= 32 || part <= (1ull << 22))) {
+        // one workgroup streams each partition: worth it with enough
+        // partitions to fill the device, or partitions short enough
+        if (roots_fit && (items >= 64 || part <= 131072)) {
             Op op;
             op.type = OP_SPARSE_STREAM;
             pl.put_runs(op, runs);
@@ -445,11 +465,11 @@ inline void plan_bst(Planner &P, uint64_t n) {
     one.p[1] = d;
     one.p[2] = K;
     P.pl.ops.push_back(one);
-    for (uint32_t t = 2 * K; t < d; t++) {
+    if (d > 2 * K) {   // phase two: every level from 2K on, in one launch
         Op two;
-        two.type = OP_BITREV;
-        two.p[0] = (1ull << t) - 1;
-        two.p[1] = t;
+        two.type = OP_BITREV_LEVELS;
+        two.p[0] = 2 * K;
+        two.p[1] = d;
         two.p[2] = K;
         P.pl.ops.push_back(two);
     }
@@ -506,7 +526,7 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
                 // the roots past the full blocks (whose bottom trees are the
                 // incomplete one and empty ones) follow the incomplete block;
                 // when the gather is a chained pass they ride along in it
-                const bool chained = nf && !(nf * Q.W <= 65536 && nf * (l + 1) <= (1ull << 22));
+                const bool chained = nf && !(nf * Q.W <= 65536 && nf * (l + 1) <= 131072);
                 if (chained && r <= Q.tu.row_stash_max) {
                     Q.sparse_slide(off, nf, l, inc, r - nf);
                 } else {
@@ -566,6 +586,12 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             const uint32_t D = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
             if (base < -1 || D < 2 * K || D > 62) neg = true;
             end = (uint64_t)(base + (int64_t)(1ull << D));
+            break;
+        }
+        case OP_BITREV_LEVELS: {
+            const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
+            if (t0 != 2 * K || d <= t0 || d > 62) neg = true;
+            end = (1ull << d) - 1;
             break;
         }
         case OP_UNIT_MERGE: {
