@@ -267,6 +267,8 @@ __device__ inline uint64_t start_slot(uint64_t lo, uint64_t hi, const BtreeConst
     return (nodestart[h] + div_pw<FAST>(plo, h, pw, mag)) * c.b;
 }
 
+constexpr int START_QPL = 4;   // start-pass queries per lane
+
 // Pass 1: the start slot of every query, one lane per query, written into the
 // answer array (no dependent chain, so the model loads and the index
 // arithmetic run at throughput).  Queries without a usable leaf start at the
@@ -287,24 +289,39 @@ __global__ __launch_bounds__(256) void k_learned_start(BtreeConst bc,
     __syncthreads();
     const LearnedHeader h = *hp;
     uint32_t fb = 0;
-    for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < nq;
-         qi += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t u = ukey<K>(Q[qi]);
-        const LeafModel m = leaves[leaf_of(u, h)];
-        uint64_t start = 0;
-        if (m.count) {
-            const int64_t pred = predict(u, m);
-            int64_t lo = pred - (int64_t)m.err, hi = pred + (int64_t)m.err;
-            lo = max(lo, (int64_t)m.base_rank);
-            hi = min(hi, (int64_t)(m.base_rank + m.count - 1));
-            if (lo <= hi)
-                start = start_slot<FAST>((uint64_t)lo, (uint64_t)hi, bc, s_pw, s_mag, s_ns);
-            else
-                fb++;
-        } else {
-            fb++;
+    // START_QPL queries per lane: their model loads are in flight together
+    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
+         q0 += all * START_QPL) {
+        uint64_t u[START_QPL];
+        LeafModel m[START_QPL];
+#pragma unroll
+        for (int k = 0; k < START_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            u[k] = qi < nq ? ukey<K>(Q[qi]) : 0;
         }
-        ans[qi] = start;
+#pragma unroll
+        for (int k = 0; k < START_QPL; k++)
+            if (q0 + k * all < nq) m[k] = leaves[leaf_of(u[k], h)];
+#pragma unroll
+        for (int k = 0; k < START_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            if (qi >= nq) continue;
+            uint64_t start = 0;
+            if (m[k].count) {
+                const int64_t pred = predict(u[k], m[k]);
+                int64_t lo = pred - (int64_t)m[k].err, hi = pred + (int64_t)m[k].err;
+                lo = max(lo, (int64_t)m[k].base_rank);
+                hi = min(hi, (int64_t)(m[k].base_rank + m[k].count - 1));
+                if (lo <= hi)
+                    start = start_slot<FAST>((uint64_t)lo, (uint64_t)hi, bc, s_pw, s_mag, s_ns);
+                else
+                    fb++;
+            } else {
+                fb++;
+            }
+            ans[qi] = start;
+        }
     }
     if (stats) {
         for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
@@ -436,8 +453,9 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
     const bool fast = n < (1ull << 32);
     const LearnedHeader *hp = (const LearnedHeader *)model;
     const LeafModel *leaves = (const LeafModel *)((const char *)model + sizeof(LearnedHeader));
-    const uint64_t sgrid = std::max<uint64_t>(1, std::min<uint64_t>((nq + 255) / 256,
-                                                                     (uint64_t)c->num_cus * 16));
+    const uint64_t sgrid = std::max<uint64_t>(
+        1, std::min<uint64_t>((nq + 256 * START_QPL - 1) / (256 * START_QPL),
+                              (uint64_t)c->num_cus * 16));
     if (fast)
         hipLaunchKernelGGL((k_learned_start<K, true>), dim3((uint32_t)sgrid), dim3(256), 0, s, bc, hp,
                            leaves, (const K *)q, nq, ans, stats);
