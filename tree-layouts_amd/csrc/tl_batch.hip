@@ -238,7 +238,7 @@ __device__ void ex_copy(const BOp &op, uint64_t sub, T *stash) {
     }
 }
 
-constexpr int MOVE_PER = 16;
+constexpr int MOVE_PER = CHAINED_PER;
 
 template <typename T>
 __device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32_t *err,

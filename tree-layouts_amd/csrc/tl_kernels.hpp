@@ -113,6 +113,10 @@ __global__ __launch_bounds__(256) void k_bitrev_levels(T *__restrict__ A, uint32
 //     rows, writes their keys contiguously and their separators to the stash.
 enum : uint32_t { MV_SHIFT_RIGHT = 0, MV_SHIFT_LEFT = 1, MV_SPARSE_SLIDE = 2, MV_ROW_SLIDE = 3 };
 
+// keys per thread of a chained move (256 threads per move)
+constexpr int CHAINED_PER = 32;
+constexpr uint64_t CHAINED_KEYS = 256 * CHAINED_PER;
+
 struct MoveSpec {
     uint32_t kind;
     uint64_t P;       // region start

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
 
 // reverse every segment [off, off+len): REV_U mirrored pairs per thread per
 // step, all loads issued before any store (the pairs are disjoint)
-constexpr int REV_U = 8;
+constexpr int REV_U = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_reverse_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
@@ -308,7 +308,7 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             if (!preclear) TL_HIP(hipMemsetAsync(op_flags, 0, (words + 16) * 4, s));
             unsigned long long *counter = (unsigned long long *)(op_flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
-            hipLaunchKernelGGL((k_chained<T, 16>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
+            hipLaunchKernelGGL((k_chained<T, CHAINED_PER>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
                                op_flags, counter, c->err_flag, c->wait_ticks, stash);
             TL_LAUNCH_CHECK();
             if (preclear) op_flags += words + 16;

@@ -217,7 +217,7 @@ struct Planner {
         const uint64_t small = std::min(k, len - k);
         if (items == 1 && small <= tu.stash_max && len > 4 * small) {
             const uint64_t P = runs[0].start;
-            const uint64_t CH = 256 * 16;
+            const uint64_t CH = CHAINED_KEYS;
             if (k <= len - k) {
                 // [X][Y] -> [Y][X] with Y small: stash Y, slide X right by k
                 copy(P + len - k, 1, k, /*to_stash=*/true);
@@ -291,7 +291,7 @@ struct Planner {
         op.ms.P = P;
         op.ms.r = r;
         op.ms.l = l;
-        op.ms.CH = std::max<uint64_t>(1, (256 * 16) / (l + 1));
+        op.ms.CH = std::max<uint64_t>(1, (CHAINED_KEYS) / (l + 1));
         op.ms.nchunk = (r + op.ms.CH - 1) / op.ms.CH;
         op.p[0] = op.ms.nchunk;
         pl.max_moves = std::max(pl.max_moves, op.p[0]);
@@ -400,7 +400,7 @@ struct Planner {
         op.ms.l = l;
         op.ms.L = tail;
         op.ms.k = extra;
-        op.ms.CH = std::min<uint64_t>(l, 256 * 16);
+        op.ms.CH = std::min<uint64_t>(l, CHAINED_KEYS);
         op.ms.nchunk = (l + op.ms.CH - 1) / op.ms.CH;
         op.p[0] = (tail + op.ms.CH - 1) / op.ms.CH + r * op.ms.nchunk;
         pl.max_moves = std::max(pl.max_moves, op.p[0]);
@@ -611,12 +611,12 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             const MoveSpec &m = op.ms;
             if (m.kind == MV_SPARSE_SLIDE) {
                 end = m.P + m.r * (m.l + 1) + m.L + m.k;
-                if (m.CH == 0 || m.CH > 256 * 16 || m.CH > m.l ||
+                if (m.CH == 0 || m.CH > CHAINED_KEYS || m.CH > m.l ||
                     op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
                     neg = true;
             } else if (m.kind == MV_ROW_SLIDE) {
                 end = m.P + m.r * (m.l + 1);
-                if (m.CH == 0 || m.CH * (m.l + 1) > 256 * 16 || m.nchunk * m.CH < m.r ||
+                if (m.CH == 0 || m.CH * (m.l + 1) > CHAINED_KEYS || m.nchunk * m.CH < m.r ||
                     m.r > pl.max_stash || op.p[0] != m.nchunk)
                     neg = true;
                 break;
@@ -626,7 +626,7 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
                 end = m.P + m.L;
                 if (m.k > m.P) neg = true;
             }
-            if (m.CH == 0 || m.CH > 256 * 16) neg = true;
+            if (m.CH == 0 || m.CH > CHAINED_KEYS) neg = true;
             break;
         }
         }
