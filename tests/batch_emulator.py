@@ -20,6 +20,7 @@ REV_CHUNK = 256 * 8
 COPY_CHUNK = 256 * 8
 QCHUNK = 4096
 THREADS = 256
+CHAINED_KEYS = 256 * 32   # keys of one chained move (32 per thread)
 
 
 class MoveSpec(ctypes.Structure):
@@ -269,7 +270,7 @@ class Replay:
         elif t == pe.OP_CHAINED:
             m = op.ms
             src, dst, ln = move_at(m, sub)
-            assert 0 < ln <= THREADS * 16
+            assert 0 < ln <= CHAINED_KEYS
             for tt in awaited_moves(m, sub, dst, ln):
                 assert self.flags.get((oid, tt)), f"move {sub} waits on unpublished {tt}"
             v = A[src:src + ln].copy()
