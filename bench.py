@@ -149,7 +149,7 @@ def run_batch(args, dev):
     # tl_batch_run: every plan built and uploaded at create time, the arrays
     # that fit in LDS in batched launches, one HIP graph replayed per step
     t0 = time.perf_counter()
-    B = tl_amd.Batch(arrays)
+    B = tl_amd.Batch(arrays, lds_bytes=args.batch_lds)
     create_s = time.perf_counter() - t0
 
     def restore():
@@ -188,7 +188,7 @@ def run_batch(args, dev):
     log(f"[bench] per-array calls: {sep_ms:.2f} ms")
     # the permutes alone, through a batch without queries
     perm_only = [dict(a, Q=None) for a in arrays]
-    P = tl_amd.Batch(perm_only)
+    P = tl_amd.Batch(perm_only, lds_bytes=args.batch_lds)
     restore()
     P.run(graph=True)
     ptimes = []
@@ -209,7 +209,8 @@ def run_batch(args, dev):
         "config": {"workload": f"uneven batch: {count} arrays, sizes floor(2^U) U~U[8,22], "
                                "BST / B-tree b=8 / mvEB round robin, 4*size hit queries each; "
                                "tl_batch_run replayed from its HIP graph",
-                   "keys": keys_total, "queries": 4 * keys_total},
+                   "keys": keys_total, "queries": 4 * keys_total,
+                   "lds_bytes": args.batch_lds or "device budget"},
         "per_array_calls_ms": round(sep_ms, 3), "create_s": round(create_s, 3),
         "queries": {"Mq_s": round(4 * keys_total / (ms * 1e-3) / 1e6, 1)},
         "permutes_alone": {"ms": round(perm_ms, 3),
@@ -319,6 +320,8 @@ def main():
     ap.add_argument("--log2leaves", type=int, default=18, help="learned model leaves (log2)")
     ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb"], default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--batch-lds", type=int, default=0,
+                    help="LDS budget of the batch's plans in bytes (0: the library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

@@ -138,7 +138,8 @@ typedef struct {
     uint64_t *ans;
 } tl_batch_item;
 
-/* lds_bytes: LDS budget of the in-LDS sub-layouts (0 = 64 KiB);
+/* lds_bytes: LDS budget of the in-LDS sub-layouts (0 = the device's whole
+ * budget, 160 KiB less a 2 KiB reserve on gfx950);
  * wg_per_cu: persistent workgroups per compute unit (0 = 4);
  * wait_ms: bound on any single in-kernel wait of the persistent launch
  * (0 = the library default, 4000 ms or $TL_WAIT_MS); a wait that reaches it
@@ -160,8 +161,9 @@ tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms);
 tl_status tl_batch_run_separate(void *handle, void *stream, float *ms);
 /* Launch plan size: number of ops, of sub-tasks and the LDS bytes per workgroup. */
 tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks, uint64_t *smem_bytes);
-/* Times candidate configurations on device copies of up to `sample` items of
- * the batch and returns the fastest in *best. */
+/* Times candidate LDS budgets (32, 64, 96 and 160 KiB) on device copies of up
+ * to `sample` items of the batch and returns the fastest configuration in
+ * *best (wg_per_cu 4, wait_ms 0). */
 tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
                             void *stream, tl_batch_config *best, float *best_ms);
 tl_status tl_batch_destroy(void *handle);

@@ -516,6 +516,7 @@ struct Batch {
     std::vector<SmallGroup> small;
     std::vector<LargeItem> large;
     std::vector<QTables> qt;      // per item (layout 2 with queries)
+    uint32_t plan_lds = 0;        // LDS budget of the plans; 0 = the device's whole budget
     void *d_pdata = nullptr, *d_pstash = nullptr;
     uint32_t *d_pflags = nullptr;
     uint64_t small_items = 0;
@@ -778,8 +779,7 @@ static tl_status upload_batch(Batch &B) {
 
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
-    if (B.cfg.lds_bytes)
-        tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
+    if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;
     auto put = [&data](const void *src, size_t bytes) -> uint64_t {
         const uint64_t at = (data.size() + 63) & ~63ull;
@@ -969,7 +969,8 @@ extern "C" tl_status tl_batch_create(const tl_batch_item *items, uint32_t count,
     TL_TRY(get_ctx(&c));
     auto B = std::make_unique<Batch>();
     B->items.assign(items, items + count);
-    B->cfg = cfg ? *cfg : tl_batch_config{65536, 4, 0};
+    B->cfg = cfg ? *cfg : tl_batch_config{0, 4, 0};
+    B->plan_lds = B->cfg.lds_bytes;
     if (B->cfg.lds_bytes == 0) B->cfg.lds_bytes = 65536;
     if (B->cfg.wg_per_cu == 0) B->cfg.wg_per_cu = 4;
     TL_TRY(build_batch(*B, c));
@@ -1143,30 +1144,31 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
     }
     float best_t = 1e30f;
     tl_status result = TL_OK;
-    for (uint32_t lds : {32768u, 65536u, 98304u})
-        for (uint32_t wpc : {2u, 4u, 8u}) {
-            tl_batch_config cfg{lds, wpc, 0};
-            void *h = nullptr;
-            if (tl_batch_create(pick.data(), (uint32_t)pick.size(), &cfg, &h) != TL_OK) continue;
-            float t = 0.f, tsum = 0.f;
-            bool ok = true;
-            for (int rep = 0; rep < 3 && ok; rep++) {
-                for (size_t k = 0; k < pick.size(); k++) {
-                    const size_t bytes =
-                        pick[k].n * ((pick[k].dtype == TL_U32 || pick[k].dtype == TL_I32) ? 4 : 8);
-                    if (hipMemcpyAsync(work[k], pristine[k], bytes, hipMemcpyDeviceToDevice, s) !=
-                        hipSuccess)
-                        ok = false;
-                }
-                if (tl_batch_run(h, stream, 0, &t) != TL_OK) ok = false;
-                if (rep) tsum += t;
+    // runs take the planned path, whose only knob is the LDS budget of the
+    // plans (the tile of whole-array LDS permutes and the run packing)
+    for (uint32_t lds : {32768u, 65536u, 98304u, 163840u}) {
+        tl_batch_config cfg{lds, 4, 0};
+        void *h = nullptr;
+        if (tl_batch_create(pick.data(), (uint32_t)pick.size(), &cfg, &h) != TL_OK) continue;
+        float t = 0.f, tsum = 0.f;
+        bool ok = true;
+        for (int rep = 0; rep < 3 && ok; rep++) {
+            for (size_t k = 0; k < pick.size(); k++) {
+                const size_t bytes =
+                    pick[k].n * ((pick[k].dtype == TL_U32 || pick[k].dtype == TL_I32) ? 4 : 8);
+                if (hipMemcpyAsync(work[k], pristine[k], bytes, hipMemcpyDeviceToDevice, s) !=
+                    hipSuccess)
+                    ok = false;
             }
-            tl_batch_destroy(h);
-            if (ok && tsum / 2 < best_t) {
-                best_t = tsum / 2;
-                *best = cfg;
-            }
+            if (tl_batch_run(h, stream, 0, &t) != TL_OK) ok = false;
+            if (rep) tsum += t;
         }
+        tl_batch_destroy(h);
+        if (ok && tsum / 2 < best_t) {
+            best_t = tsum / 2;
+            *best = cfg;
+        }
+    }
     for (void *p : pristine) (void)hipFree(p);
     for (void *p : work) (void)hipFree(p);
     for (void *p : answers) (void)hipFree(p);
