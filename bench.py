@@ -384,6 +384,20 @@ def main():
         torch.equal(A[ans.clamp(0, n - 1)], Q))
     log(f"[bench] plain query {q_ms:.3f} ms ({nq / q_ms / 1e3:.1f} Mq/s) ok={plain_ok}")
 
+    # ---- the comparator: binary search of the sorted keys, same queries
+    for _ in range(max(1, args.warmup)):
+        tl_amd.query_binary(keys, Q, ans, kind="u64", timed=False)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        tl_amd.query_binary(keys, Q, ans, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    bs_ms = ev0.elapsed_time(ev1) / args.steps
+    bs_ok = bool(((ans >= 0) & (ans < n)).all().item()) and bool(
+        torch.equal(keys[ans.clamp(0, n - 1)], Q))
+    log(f"[bench] binary search {bs_ms:.3f} ms ({nq / bs_ms / 1e3:.1f} Mq/s) ok={bs_ok}")
+
     learned = None
     if not args.skip_learned and hasattr(tl_amd, "learned_fit"):
         learned = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n,
@@ -419,7 +433,9 @@ def main():
                                "2^27 uniform-hit queries (plain and learned start)",
                    "n": n, "b": b, "queries": nq},
         "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
-                    "learned": learned},
+                    "learned": learned,
+                    "binary_search": {"ms": round(bs_ms, 4), "Mq_s": round(nq / bs_ms / 1e3, 2),
+                                      "answers_ok": bs_ok}},
         "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

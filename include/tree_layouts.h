@@ -85,6 +85,14 @@ tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_ta
                                    const void *dev_q, uint64_t nq, uint64_t *dev_ans,
                                    void *stream, float *ms);
 
+/* Binary search of the sorted dev_A[0..n), the layouts' comparator.  Replaces
+ * searchAll / timeQuery of gpu/src/binary-search.cuh:52,63 (CPU twin
+ * cpu/src/binary-search.h:55,64).  For distinct keys dev_ans[i] is the index
+ * of q[i], or of its predecessor when q[i] is absent, or n when every key is
+ * larger than q[i] (there the reference's loop does not end). */
+tl_status tl_query_binary(const void *dev_A, uint64_t n, tl_dtype dtype, const void *dev_q,
+                          uint64_t nq, uint64_t *dev_ans, void *stream, float *ms);
+
 /* ---- mvEB query tables -------------------------------------------------- */
 
 /* Host table build.  Replaces buildTable(vEB_table*, n, d, root_depth)

@@ -4,7 +4,7 @@
 // (README.md:31-67; gpu/src/bst-hybrid.cuh:120, gpu/src/btree-cycles.cuh:77,
 // gpu/src/mveb-cycles.cuh:420, gpu/src/query-bst.cuh:57,
 // gpu/src/query-btree.cuh:76, gpu/src/query-mveb.cuh:86,231,
-// gpu/src/query-mveb.cu:25), so a reference driver switches by changing its
+// gpu/src/query-mveb.cu:25, gpu/src/binary-search.cuh:63), so a reference driver switches by changing its
 // includes.  Differences, all deliberate:
 //   * one header for every layout (the reference needs one translation unit
 //     per layout header because their helper templates collide);
@@ -124,6 +124,17 @@ float timeQueryBtree(T *A, T *dev_A, uint64_t n, uint64_t b, uint64_t q, uint64_
         return tl_detail::ok(
             tl_query_btree(dev_A, n, b, tl_detail::dtype_of<T>::v, dq, q, dans, nullptr, t),
             "timeQueryBtree");
+    });
+}
+
+// binary search of the sorted array (gpu/src/binary-search.cuh:63): here A
+// and dev_A hold the sorted keys, not a layout
+template <typename T>
+float timeQuery(T *A, T *dev_A, uint64_t n, uint64_t q, uint64_t seed = 0) {
+    return tl_detail::run_queries(A, dev_A, n, q, seed, [&](T *dq, uint64_t *dans, float *t) {
+        return tl_detail::ok(
+            tl_query_binary(dev_A, n, tl_detail::dtype_of<T>::v, dq, q, dans, nullptr, t),
+            "timeQuery");
     });
 }
 

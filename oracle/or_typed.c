@@ -19,7 +19,9 @@
 #undef SUF
 
 /* ---- queries ------------------------------------------------------------
- * All three return the slot of the query key, or n when it is absent.     */
+ * The three layout searches return the slot of the query key, or n when it
+ * is absent; the binary search of the sorted array answers misses with the
+ * predecessor's index.                                                     */
 
 #define OR_QUERIES(SUF, T)                                                                     \
     /* cpu/src/query-bst.h:30-43 */                                                            \
@@ -83,6 +85,30 @@
             offset = idx[j];                                                                   \
         }                                                                                      \
         return n;                                                                              \
+    }                                                                                          \
+    /* cpu/src/binary-search.h:30-50 on the sorted array: a hit answers the    \
+     * key's index, a miss the index of its predecessor (`right` at the end of \
+     * the loop).  With no predecessor (q < A[0], or n = 0) the reference's    \
+     * unsigned `right` wraps and its loop never ends; here that answers n. */ \
+    static uint64_t binary_one_##SUF(const T *A, uint64_t n, T q) {                            \
+        if (n == 0) return n;                                                                  \
+        uint64_t left = 0, right = n - 1;                                                      \
+        while (left <= right) {                                                                \
+            const uint64_t mid = (left + right + 1) / 2;                                       \
+            if (q == A[mid]) return mid;                                                       \
+            if (q > A[mid]) {                                                                  \
+                left = mid + 1;                                                                \
+            } else {                                                                           \
+                if (mid == 0) return n;                                                        \
+                right = mid - 1;                                                               \
+            }                                                                                  \
+        }                                                                                      \
+        return right;                                                                          \
+    }                                                                                          \
+    void or_search_binary_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,               \
+                                uint64_t *ans, int p) {                                        \
+        _Pragma("omp parallel for schedule(guided) num_threads(p) if (p > 1)")                 \
+        for (uint64_t i = 0; i < nq; i++) ans[i] = binary_one_##SUF(A, n, q[i]);              \
     }                                                                                          \
     void or_search_mveb_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq, uint64_t *ans,  \
                               int p) {                                                         \

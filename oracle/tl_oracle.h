@@ -73,7 +73,10 @@ int or_permute_mveb_u64(uint64_t *A, uint64_t n);
     void or_search_btree_##SUF(const T *A, uint64_t n, uint64_t b, const T *q, uint64_t nq, \
                                uint64_t *ans, int p);                                       \
     void or_search_mveb_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,              \
-                              uint64_t *ans, int p);
+                              uint64_t *ans, int p);                                        \
+    /* sorted A: hit -> index, miss -> predecessor's index, none -> n */                    \
+    void or_search_binary_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,            \
+                                uint64_t *ans, int p);
 OR_QDECL(u32, uint32_t)
 OR_QDECL(u64, uint64_t)
 OR_QDECL(i32, int32_t)

@@ -1,5 +1,6 @@
 // A reference-style driver written against include/tree_layouts.hpp only
-// (compare gpu/test/query-btree-fixedN.cu and gpu/test/query-mveb-fixedN.cu):
+// (compare gpu/test/query-btree-fixedN.cu, gpu/test/query-mveb-fixedN.cu and
+// gpu/test/query-binarysearch-fixedN.cu):
 // build each layout on the GPU, copy it back, and query it through the
 // reference-named templates.  Exit status 0 when every check passes.
 #include <stdio.h>
@@ -61,6 +62,12 @@ static int run(uint64_t n) {
         printf("layout %d n=%llu permute %.3f ms, 100000 queries %.3f ms\n", layout,
                (unsigned long long)n, ms, qms);
     }
+    // the comparator on the sorted list (gpu/test/query-binarysearch-fixedN.cu)
+    for (uint64_t i = 0; i < n; i++) A[i] = (T)(2 * i + 1);
+    if (hipMemcpy(dev_A, A.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    const float bms = timeQuery<T>(A.data(), dev_A, n, 100000, 7);
+    if (bms < 0) bad++;
+    printf("binary search n=%llu, 100000 queries %.3f ms\n", (unsigned long long)n, bms);
     (void)hipFree(dev_A);
     return bad;
 }

@@ -7,7 +7,8 @@ Fixtures are data only: rank layouts (the slot -> rank map of permuting
 Layouts do not depend on key values, so a rank layout fixes the layout of any
 sorted key set of that size.
 
-Run:  python tests/golden/make_golden.py
+Run:  python tests/golden/make_golden.py           (every fixture)
+      python tests/golden/make_golden.py binary    (binary_search.npz only)
 """
 import json
 import os
@@ -52,9 +53,35 @@ def edge_sizes(b, limit):
 REF_CRASH = {(32, 33 ** 3)}
 
 
+def binary_search_fixtures():
+    """Answers of the reference's CPU binary search (cpu/src/binary-search.h)
+    on sorted arrays: hits, and misses above the smallest key (below it the
+    reference's loop does not end, so those are not asked)."""
+    r = ol.ref()
+    rng = np.random.default_rng(20240602)
+    out = {}
+    for n in (1, 2, 3, 7, 8, 1000, 4096, 5001):
+        keys = np.sort(rng.choice(np.arange(1, 10 * n + 10, dtype=np.uint64), size=n,
+                                  replace=False))
+        hits = keys[rng.integers(0, n, size=1000)]
+        miss = rng.integers(int(keys[0]), 10 * n + 20, size=1000).astype(np.uint64)
+        q = np.concatenate([hits, miss])
+        ans = np.zeros(q.size, dtype=np.uint64)
+        r.ref_search_binary_u64(keys.ctypes.data_as(ol.VP), n, q.ctypes.data_as(ol.VP), q.size,
+                                ans.ctypes.data_as(ol.VP), 1)
+        out[f"binary_n{n}_A"] = keys
+        out[f"binary_n{n}_q"] = q
+        out[f"binary_n{n}_ans"] = ans
+    np.savez_compressed(os.path.join(HERE, "binary_search.npz"), **out)
+    print("wrote", len(out) // 3, "binary-search cases")
+
+
 def main():
     if ol.ref() is None:
         sys.exit("oracle/_ref/libref.so is not built (make -C oracle ref)")
+    binary_search_fixtures()
+    if sys.argv[1:] == ["binary"]:
+        return
     small = {}
     for layout, b in [("bst", 1), ("btree", 2), ("btree", 3), ("btree", 8), ("btree", 16),
                       ("btree", 32)]:

@@ -49,6 +49,7 @@ def oracle():
             getattr(o, f"or_search_bst_{k}").argtypes = [VP, C, VP, C, VP, I]
             getattr(o, f"or_search_btree_{k}").argtypes = [VP, C, C, VP, C, VP, I]
             getattr(o, f"or_search_mveb_{k}").argtypes = [VP, C, VP, C, VP, I]
+            getattr(o, f"or_search_binary_{k}").argtypes = [VP, C, VP, C, VP, I]
         _o = o
     return _o
 
@@ -65,6 +66,7 @@ def ref():
         r.ref_permute_veb_u64.argtypes = [VP, C, I]
         r.ref_search_bst_u64.argtypes = [VP, C, VP, C, VP, I]
         r.ref_search_btree_u64.argtypes = [VP, C, C, VP, C, VP, I]
+        r.ref_search_binary_u64.argtypes = [VP, C, VP, C, VP, I]
         r.ref_build_table_classic.argtypes = [VP, C, ctypes.c_uint32]
         _r = r
     return _r
@@ -128,6 +130,8 @@ def search(layout, A, Q, kind, b=16, p=8):
         getattr(o, f"or_search_bst_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     elif layout == "btree":
         getattr(o, f"or_search_btree_{kind}")(_p(A), A.size, b, _p(Q), Q.size, _p(ans), p)
+    elif layout == "binary":   # A is the sorted array itself
+        getattr(o, f"or_search_binary_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     else:
         getattr(o, f"or_search_mveb_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     return ans

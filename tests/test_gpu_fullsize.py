@@ -124,8 +124,9 @@ def test_config3_mveb(n, dev):
 
 
 def test_config4_btree_2p30_learned(dev):
-    """n = 2^30 uint64 (lognormal(0, 2) gaps), b = 16: the layout, and 2^24
-    uniform hits answered by the plain and the learned-start descent alike."""
+    """n = 2^30 uint64 (lognormal(0, 2) gaps), b = 16: the layout, 2^24
+    uniform hits answered by the plain and the learned-start descent alike,
+    and the binary search of the sorted keys on the same hits and on key + 1."""
     import bench
     n, b = 1 << 30, 16
     keys, g = bench.gen_keys(n, 4, dev)
@@ -134,7 +135,20 @@ def test_config4_btree_2p30_learned(dev):
     assert _sorted_back_equal(A, keys)
     assert _sample_slots("btree", A, keys, n, b, np.random.default_rng(4))
     nq = 1 << 24
-    Q = keys[torch.randint(0, n, (nq,), generator=g, device=dev, dtype=torch.int64)]
+    qi = torch.randint(0, n, (nq,), generator=g, device=dev, dtype=torch.int64)
+    Q = keys[qi]
+    # the comparator on the sorted keys: hits answer their index, key + 1
+    # answers its predecessor (the key itself, or the next one when the gap is 1)
+    bs = torch.empty(nq, dtype=torch.int64, device=dev)
+    tl_amd.query_binary(keys, Q, bs, kind="u64")
+    assert torch.equal(bs, qi)
+    Qm = Q + 1
+    tl_amd.query_binary(keys, Qm, bs, kind="u64")
+    assert bool(((bs >= 0) & (bs < n)).all())
+    assert bool((keys[bs] <= Qm).all())
+    nxt = (bs + 1).clamp(max=n - 1)
+    assert bool(((bs == n - 1) | (keys[nxt] > Qm)).all())
+    del Qm, bs, qi
     plain = torch.empty(nq, dtype=torch.int64, device=dev)
     tl_amd.query_btree(A, b, Q, plain, kind="u64")
     _check_answers(A, Q, plain, n, np.ones(nq, dtype=bool))

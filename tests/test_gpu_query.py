@@ -84,6 +84,30 @@ def test_query_matches_oracle(layout, b, kind, dev):
         assert not np.isin(Q[~hit], keys).any()
 
 
+@pytest.mark.parametrize("kind", ["u32", "u64", "i32", "i64"])
+def test_binary_search_matches_oracle(kind, dev):
+    """Sorted array: hit -> index, miss -> predecessor's index, below the
+    smallest key -> n; sizes around the 2047-key LDS top and its levels."""
+    rng = np.random.default_rng(4321)
+    for n in [0] + SIZES + [2047, 2048, 2049, 4094]:
+        keys = make_keys(kind, n, rng) if n else np.zeros(0, dtype=NP[kind])
+        if n:
+            Q = make_queries(kind, keys, 4096, rng)
+            info = np.iinfo(NP[kind])
+            Q = np.concatenate([Q, np.array([info.min, info.max], dtype=NP[kind]), keys[:1] - 1
+                                if keys[0] > info.min else keys[:1]]).astype(NP[kind])
+        else:
+            Q = rng.integers(0, 1000, size=100).astype(NP[kind])
+        A = to_dev(keys, kind, dev) if n else torch.zeros(1, dtype=STORE[kind][1], device=dev)
+        Qd = to_dev(Q, kind, dev)
+        ans = torch.full((Q.size,), -7, dtype=torch.int64, device=dev)
+        tl_amd.query_binary(A[:n], Qd, ans, kind=kind)
+        torch.cuda.synchronize()
+        got = ans.cpu().numpy().view(np.uint64)
+        exp = ol.search("binary", keys, Q, kind)
+        assert np.array_equal(got, exp), (kind, n)
+
+
 def test_tableset_device_matches_oracle(dev):
     for n in [1, 2, 3, 6, 7, 100, 127, 1000, 65535, 65536, 100000, 300_000_000, (1 << 29) - 1,
               3 * 10 ** 8, (1 << 30)]:

@@ -166,6 +166,39 @@ def test_queries_brute_force_all_kinds():
                 assert np.array_equal(ans, exp), (kind, layout, b, n)
 
 
+def test_binary_search_matches_reference_fixtures():
+    data = np.load(os.path.join(GOLD, "binary_search.npz"))
+    tags = sorted({k.rsplit("_", 1)[0] for k in data.files})
+    assert len(tags) == 8
+    for tag in tags:
+        A, q, ans = data[tag + "_A"], data[tag + "_q"], data[tag + "_ans"]
+        assert np.array_equal(ol.search("binary", A, q, "u64"), ans), tag
+
+
+def test_binary_search_brute_force_all_kinds():
+    """Hit -> index, miss -> index of the predecessor (numpy's searchsorted
+    from the right, minus one), no predecessor -> n; n = 0 answers 0."""
+    rng = np.random.default_rng(12)
+    for kind, dt in (("u32", np.uint32), ("u64", np.uint64), ("i32", np.int32), ("i64", np.int64)):
+        for n in (0, 1, 2, 3, 9, 100, 777, 4096):
+            lo = -1000 if kind[0] == "i" else 5
+            keys = np.sort(rng.choice(np.arange(lo, lo + 10 * n + 1), size=n,
+                                      replace=False)).astype(dt)
+            q = np.arange(lo - 5, lo + 10 * n + 6).astype(dt)
+            ans = ol.search("binary", keys, q, kind)
+            pred = np.searchsorted(keys, q, side="right").astype(np.int64) - 1
+            exp = np.where(pred < 0, n, pred).astype(np.uint64)
+            assert np.array_equal(ans, exp), (kind, n)
+    # extremes of the key range
+    for kind, dt in (("u64", np.uint64), ("i64", np.int64), ("u32", np.uint32), ("i32", np.int32)):
+        info = np.iinfo(dt)
+        keys = np.array([info.min, info.min + 1, 0 if info.min < 0 else 7, info.max - 1, info.max],
+                        dtype=dt)
+        q = np.array([info.min, info.max, info.max - 2, info.min + 2], dtype=dt)
+        exp = np.searchsorted(keys, q, side="right") - 1
+        assert np.array_equal(ol.search("binary", keys, q, kind), exp.astype(np.uint64)), kind
+
+
 def test_parallel_restatement_equals_serial():
     for layout, b in (("bst", 1), ("btree", 16), ("btree", 8)):
         for n in (1000, 65536, 100003, 1 << 20):
@@ -174,6 +207,20 @@ def test_parallel_restatement_equals_serial():
             ol.permute(layout, a1, b, p=1)
             ol.permute(layout, a8, b, p=8)
             assert np.array_equal(a1, a8), (layout, b, n)
+
+
+@pytest.mark.skipif(ol.ref() is None, reason="reference not compiled in this environment")
+def test_live_reference_binary_search_agrees():
+    rng = np.random.default_rng(13)
+    for n in (1, 2, 5, 64, 1000, 65537, 1 << 20):
+        keys = np.sort(rng.choice(np.arange(1, 4 * n + 10, dtype=np.uint64), size=n,
+                                  replace=False))
+        q = np.concatenate([keys[rng.integers(0, n, size=3000)],
+                            rng.integers(int(keys[0]), 4 * n + 20, size=3000).astype(np.uint64)])
+        ans = np.zeros(q.size, dtype=np.uint64)
+        ol.ref().ref_search_binary_u64(keys.ctypes.data_as(ol.VP), n, q.ctypes.data_as(ol.VP),
+                                       q.size, ans.ctypes.data_as(ol.VP), 8)
+        assert np.array_equal(ol.search("binary", keys, q, "u64"), ans), n
 
 
 @pytest.mark.skipif(ol.ref() is None, reason="reference not compiled in this environment")

@@ -131,6 +131,103 @@ __global__ __launch_bounds__(256) void k_query_btree_lane(const K *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Binary search of the sorted array, the layouts' comparator.  For distinct
+// keys it answers as the reference's loop (cpu/src/binary-search.h:30-50,
+// gpu/src/binary-search.cuh:28-48): the key's index on a hit, the index of its
+// predecessor on a miss, and n where no key is <= q (there the reference's
+// unsigned `right` wraps and its loop does not end).
+//
+// Branch-free form: the window [base, base + len) always holds the last key
+// <= q (or base = 0 when there is none); a step probes A[base + len/2] and
+// keeps the upper or lower part.  The len sequence does not depend on q, so
+// every query takes the same ceil(log2 n) steps and a wave never diverges.
+// The probes of the first BIN_TOPL steps form a complete binary tree, staged
+// per workgroup in LDS in heap order; the answer is base when A[base] <= q.
+constexpr uint32_t BIN_TOPL = 11;   // 2047 probe keys in LDS
+constexpr int BIN_QPL = 4;          // queries in flight per lane
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_query_binary(const K *__restrict__ A, uint64_t n,
+                                                      const K *__restrict__ Q, uint64_t nq,
+                                                      uint64_t *__restrict__ ans) {
+    __shared__ K top[1u << BIN_TOPL];   // top[0] = A[0], node i >= 1 in heap order
+    __shared__ uint64_t half[64];
+    __shared__ uint32_t steps_s;
+    if (threadIdx.x == 0) {
+        uint64_t len = n;
+        uint32_t t = 0;
+        while (len > 1) {
+            half[t++] = len >> 1;
+            len -= len >> 1;
+        }
+        steps_s = t;
+    }
+    __syncthreads();
+    const uint32_t S = steps_s, T = min(S, BIN_TOPL);
+    if (n) {
+        // node i at level t probes base(i) + half[t], base(i) = the halves
+        // taken on the path from the root (the bits of i below its top bit)
+        for (uint32_t i = threadIdx.x; i < (1u << T); i += blockDim.x) {
+            if (i == 0) {
+                top[0] = A[0];
+                continue;
+            }
+            const uint32_t t = 31 - __clz(i);
+            uint64_t base = 0;
+            for (uint32_t u = 0; u < t; u++)
+                if ((i >> (t - 1 - u)) & 1) base += half[u];
+            top[i] = A[base + half[t]];
+        }
+    }
+    __syncthreads();
+    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
+         q0 += all * BIN_QPL) {
+        K qv[BIN_QPL], vb[BIN_QPL];
+        uint64_t base[BIN_QPL];
+        uint32_t node[BIN_QPL];
+#pragma unroll
+        for (int k = 0; k < BIN_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            qv[k] = qi < nq ? Q[qi] : K{};
+            base[k] = 0;
+            node[k] = 1;
+            vb[k] = n ? top[0] : K{};
+        }
+        for (uint32_t t = 0; t < T; t++) {
+            const uint64_t h = half[t];
+#pragma unroll
+            for (int k = 0; k < BIN_QPL; k++) {
+                const K v = top[node[k]];
+                const bool go = v <= qv[k];
+                node[k] = 2 * node[k] + go;
+                if (go) {
+                    base[k] += h;
+                    vb[k] = v;
+                }
+            }
+        }
+        for (uint32_t t = T; t < S; t++) {
+            const uint64_t h = half[t];
+            K v[BIN_QPL];
+#pragma unroll
+            for (int k = 0; k < BIN_QPL; k++) v[k] = A[base[k] + h];
+#pragma unroll
+            for (int k = 0; k < BIN_QPL; k++)
+                if (v[k] <= qv[k]) {
+                    base[k] += h;
+                    vb[k] = v[k];
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < BIN_QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            if (qi < nq) ans[qi] = (n && vb[k] <= qv[k]) ? base[k] : n;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // mvEB: one lane per query with the Brodal-Fagerberg-Jacob recurrence
 // pos[t] = pos[D[t]] + R[t] + (bfs & R[t]) * L[t]; pos[] lives in LDS as
 // [depth][lane] (32-bit when the layout has fewer than 2^32 keys) and the
@@ -323,6 +420,13 @@ static void launch_bst(Ctx *c, const void *A, uint64_t n, const void *q, uint64_
 }
 
 template <typename K>
+static void launch_binary(Ctx *c, const void *A, uint64_t n, const void *q, uint64_t nq,
+                          uint64_t *ans, hipStream_t s) {
+    hipLaunchKernelGGL(k_query_binary<K>, dim3(grid_for(c, nq, 256 * BIN_QPL)), dim3(256), 0, s,
+                       (const K *)A, n, (const K *)q, nq, ans);
+}
+
+template <typename K>
 static void launch_btree(Ctx *c, const void *A, uint64_t n, uint64_t b, const void *q, uint64_t nq,
                          uint64_t *ans, hipStream_t s) {
     if (b >= 2 && b <= 64) {
@@ -385,6 +489,21 @@ extern "C" tl_status tl_query_bst(const void *dev_A, uint64_t n, tl_dtype dtype,
     Timer tm;
     TL_TRY(tm.start(s, ms));
     TL_DISPATCH(dtype, launch_bst, c, dev_A, n, dev_q, nq, dev_ans, s);
+    TL_LAUNCH_CHECK();
+    return tm.stop();
+}
+
+extern "C" tl_status tl_query_binary(const void *dev_A, uint64_t n, tl_dtype dtype,
+                                     const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                                     void *stream, float *ms) {
+    TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
+    if (nq == 0) return TL_OK;
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    Timer tm;
+    TL_TRY(tm.start(s, ms));
+    TL_DISPATCH(dtype, launch_binary, c, dev_A, n, dev_q, nq, dev_ans, s);
     TL_LAUNCH_CHECK();
     return tm.stop();
 }

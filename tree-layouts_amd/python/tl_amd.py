@@ -46,6 +46,9 @@ _SIGS = {
     "tl_query_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                      ctypes.POINTER(ctypes.c_float)],
+    "tl_query_binary": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                        ctypes.POINTER(ctypes.c_float)],
     "tl_query_btree": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                        ctypes.POINTER(ctypes.c_float)],
@@ -192,6 +195,14 @@ def permute(layout, A, b=16, kind=None, stream=None, timed=True):
 def query_bst(A, Q, ans, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_bst, _ptr(A), A.numel(), DTYPE[k], _ptr(Q), Q.numel(),
+                       _ptr(ans), _stream(stream), timed=timed)
+
+
+def query_binary(A, Q, ans, kind=None, stream=None, timed=True):
+    """Binary search of the sorted A: the key's index, its predecessor's
+    index when absent, or n when every key is larger."""
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_binary, _ptr(A), A.numel(), DTYPE[k], _ptr(Q), Q.numel(),
                        _ptr(ans), _stream(stream), timed=timed)
 
 
