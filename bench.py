@@ -226,6 +226,23 @@ def run_batch(args, dev):
     B.close()
 
 
+def time_unpermute(args, A, layout, b, stream):
+    """ms per in-place inverse permutation (layout -> sorted order) of A; the
+    data movement does not depend on the values, so A's contents do not matter"""
+    import torch
+    import tl_amd
+    for _ in range(max(1, args.warmup)):
+        tl_amd.unpermute(layout, A, b=b, kind="u64", timed=False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        tl_amd.unpermute(layout, A, b=b, kind="u64", timed=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / args.steps
+
+
 def run_layout(args, dev, layout):
     """Level-order BST or mvEB at n = 2^log2n uint64 (same key generator as the
     headline): in-place permute timing and plain query timing."""
@@ -264,6 +281,8 @@ def run_layout(args, dev, layout):
     torch.cuda.synchronize()
     perm_ms = ev0.elapsed_time(ev1) / args.steps
     log(f"[bench] permute {perm_ms:.3f} ms/step")
+    unperm_ms = time_unpermute(args, A, layout, 1, stream)
+    log(f"[bench] unpermute {unperm_ms:.3f} ms/step")
     A.copy_(keys)
     tl_amd.permute(layout, A, kind="u64")
     ans = torch.empty(nq, dtype=torch.int64, device=dev)
@@ -298,6 +317,7 @@ def run_layout(args, dev, layout):
         "config": {"workload": f"level-order {layout}, in-place permute of n=2^{args.log2n} uint64 "
                                f"keys; 2^{args.log2q} uniform-hit queries", "n": n, "queries": nq},
         "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2)},
+        "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
         "checks": {"layout_sample": layout_ok, "answers": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -366,6 +386,8 @@ def main():
     wall_ms = (time.perf_counter() - t_wall) * 1e3
     perm_ms = ev0.elapsed_time(ev1) / args.steps
     log(f"[bench] permute {perm_ms:.3f} ms/step (host wall {wall_ms / args.steps:.3f})")
+    unperm_ms = time_unpermute(args, A, "btree", b, stream)
+    log(f"[bench] unpermute {unperm_ms:.3f} ms/step")
 
     # ---- rebuild the true layout for the queries
     A.copy_(keys)
@@ -432,6 +454,7 @@ def main():
         "config": {"workload": "level-order B-tree b=16, in-place permute of n=2^30 uint64 keys; "
                                "2^27 uniform-hit queries (plain and learned start)",
                    "n": n, "b": b, "queries": nq},
+        "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
         "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
                     "learned": learned,
                     "binary_search": {"ms": round(bs_ms, 4), "Mq_s": round(nq / bs_ms / 1e3, 2),

@@ -59,6 +59,18 @@ tl_status tl_permute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype, 
  * timePermutevEB<T>(T *dev_A, uint64_t n)  (gpu/src/mveb-cycles.cuh:420). */
 tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
 
+/* ---- inverse permutations: layout -> sorted order, in place ------------- */
+
+/* The exact inverses of tl_permute_{bst,btree,mveb} for the same (n, b): the
+ * key at each slot of the layout goes back to its rank, whatever the values.
+ * No counterpart in the reference, whose README (README.md:2) names the
+ * permutations "between sorted order and" the layouts but implements only
+ * the forward direction.  Same scratch bound as the forward call. */
+tl_status tl_unpermute_bst(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+tl_status tl_unpermute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype, void *stream,
+                             float *ms);
+tl_status tl_unpermute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+
 /* ---- batched queries --------------------------------------------------- */
 
 /* Replaces searchAll / timeQueryBST (gpu/src/query-bst.cuh:47,57). */
@@ -178,8 +190,9 @@ tl_status tl_batch_destroy(void *handle);
 
 /* ---- plan inspection (no device needed) -------------------------------- */
 
-/* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2): builds and bounds-
- * checks the launch plan for (n, b, dtype) with an LDS budget of lds_bytes
+/* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2; layout | 0x100 for
+ * tl_unpermute_*): builds and bounds-checks the launch plan for (n, b, dtype)
+ * with an LDS budget of lds_bytes
  * (0 = 64 KiB) and serializes it into buf (cap words; *used receives the
  * words needed).  Word layout: [ops, blob_words], then 32 words per launch,
  * then the blob of item runs and cycle tables.  Lets a test replay the plan

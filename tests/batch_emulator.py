@@ -95,6 +95,18 @@ def move_at(m, t):
         j0 = (m.nchunk - 1 - t) * m.CH
         j1 = min(m.r, j0 + m.CH)
         return m.P + j0 * (m.l + 1), m.P + m.r + j0 * m.l, (j1 - j0) * (m.l + 1)
+    if m.kind == pe.MV_ROW_SPREAD:
+        j0 = t * m.CH
+        j1 = min(m.r, j0 + m.CH)
+        return m.P + m.r + j0 * m.l, m.P + j0 * (m.l + 1), (j1 - j0) * m.l
+    if m.kind == pe.MV_SPARSE_SPREAD:
+        if t < m.r * m.nchunk:
+            i, c = divmod(t, m.nchunk)
+            return (m.P + m.r + m.k + i * m.l + c * m.CH, m.P + i * (m.l + 1) + c * m.CH,
+                    min(m.CH, m.l - c * m.CH))
+        c = t - m.r * m.nchunk
+        dst = m.P + m.r * (m.l + 1) + c * m.CH
+        return dst + m.k, dst, min(m.CH, m.L - c * m.CH)
     ntail = (m.L + m.CH - 1) // m.CH
     if t < ntail:
         c = ntail - 1 - t
@@ -120,6 +132,25 @@ def awaited_moves(m, t, dst, ln):
             if tt < t:
                 out.append(tt)
         return out
+    if m.kind in (pe.MV_ROW_SPREAD, pe.MV_SPARSE_SPREAD):
+        S0 = m.P + m.r + (m.k if m.kind == pe.MV_SPARSE_SPREAD else 0)
+        span = ln // m.l * (m.l + 1) if m.kind == pe.MV_ROW_SPREAD else ln
+        e = dst + span
+        if e <= S0:
+            return out
+        a, z = max(dst, S0) - S0, e - 1 - S0
+        if m.kind == pe.MV_ROW_SPREAD:
+            lo, hi = a // (m.CH * m.l), z // (m.CH * m.l)
+        else:
+            leaf = m.r * m.l
+
+            def num(x):
+                if x < leaf:
+                    i = x // m.l
+                    return i * m.nchunk + (x - i * m.l) // m.CH
+                return m.r * m.nchunk + (x - leaf) // m.CH
+            lo, hi = num(a), num(z)
+        return [u for u in range(lo, hi + 1) if u < t]
     if m.kind == pe.MV_ROW_SLIDE:
         dlen = ln // (m.l + 1) * m.l
         G = m.CH * (m.l + 1)

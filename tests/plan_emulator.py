@@ -12,7 +12,8 @@ import oracle_lib as ol
 (OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
  OP_BITREV_LEVELS) = range(8)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV = range(4)
-MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE = range(4)
+MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE, MV_ROW_SPREAD, MV_SPARSE_SPREAD = \
+    range(6)
 
 
 def decode(buf):
@@ -68,9 +69,9 @@ def rev_bits(y, D):
     return out
 
 
-def replay(buf, A):
+def replay(buf, A, stash=None):
     ops, blob = decode(buf)
-    stash = np.zeros(0, dtype=A.dtype)
+    stash = np.zeros(0, dtype=A.dtype) if stash is None else stash
 
     def put(off, vals):
         nonlocal stash
@@ -83,11 +84,14 @@ def replay(buf, A):
         t = int(op[0])
         p = [int(v) for v in op[3:11]]
         if t == OP_LDS:
-            n, kind, b, rows = p[0], p[1], p[2], p[3]
+            n, kind, b, rows, inv = p[0], p[1], p[2], p[3], p[5]
             pos = lds_perm(kind, n, b, rows)
             for o in items(op, blob):
                 sub = A[o:o + n].copy()
-                A[o + pos] = sub
+                if inv:
+                    A[o:o + n] = sub[pos]
+                else:
+                    A[o + pos] = sub
         elif t == OP_REVERSE:
             ln = p[0]
             for o in items(op, blob):
@@ -108,7 +112,7 @@ def replay(buf, A):
                 old = A.copy()
                 A[idx] = old[par]
         elif t == OP_UNIT_MERGE:
-            unit, cs_at, ncyc, cu_at = p[0], p[1], p[2], p[3]
+            unit, cs_at, ncyc, cu_at, inv = p[0], p[1], p[2], p[3], p[4]
             cs = blob[cs_at: cs_at + ncyc + 1]
             for o in items(op, blob):
                 old = None
@@ -116,11 +120,18 @@ def replay(buf, A):
                     us = blob[cu_at + cs[c]: cu_at + cs[c + 1]]
                     vals = [A[o + u * unit: o + (u + 1) * unit].copy() for u in us]
                     for k in range(len(us)):
-                        dst = us[(k + 1) % len(us)]
+                        dst = us[(k - 1) % len(us)] if inv else us[(k + 1) % len(us)]
                         A[o + dst * unit: o + (dst + 1) * unit] = vals[k]
         elif t == OP_SPARSE_STREAM:
-            r, l = p[0], p[1]
+            r, l, inv = p[0], p[1], p[2]
             for o in items(op, blob):
+                if inv:
+                    roots = A[o:o + r].copy()
+                    leaves = A[o + r: o + r * (l + 1)].reshape(r, l).copy()
+                    reg = A[o:o + r * (l + 1)].reshape(r, l + 1)
+                    reg[:, :l] = leaves
+                    reg[:, l] = roots
+                    continue
                 reg = A[o:o + r * (l + 1)].reshape(r, l + 1).copy()
                 A[o:o + r] = reg[:, l]
                 A[o + r: o + r * (l + 1)] = reg[:, :l].reshape(-1)
@@ -136,6 +147,19 @@ def replay(buf, A):
                 A[P + k: P + k + L] = A[P:P + L].copy()
             elif kind == MV_SHIFT_LEFT:
                 A[P - k: P - k + L] = A[P:P + L].copy()
+            elif kind == MV_ROW_SPREAD:
+                # r rows of l keys back into rows of (l keys, separator from
+                # the stash)
+                keys = A[P + r: P + r + r * l].reshape(r, l).copy()
+                reg = A[P:P + r * (l + 1)].reshape(r, l + 1)
+                reg[:, :l] = keys
+                reg[:, l] = stash[:r]
+            elif kind == MV_SPARSE_SPREAD:
+                T0 = P + r * (l + 1)
+                leaves = A[P + r + k: P + r + k + r * l].reshape(r, l).copy()
+                tail = A[T0 + k: T0 + k + L].copy()
+                A[P:P + r * (l + 1)].reshape(r, l + 1)[:, :l] = leaves
+                A[T0:T0 + L] = tail
             elif kind == MV_ROW_SLIDE:
                 # rows of (l keys, 1 separator): separators to the stash
                 reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()

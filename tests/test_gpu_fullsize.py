@@ -4,7 +4,8 @@ Where the oracle finishes in seconds (config 1) the whole layout and every
 answer are compared with it.  At 10^8 .. 2^30 keys the checks are
 size-independent: the layout is a permutation of the input (sorted back on the
 GPU it equals the input), 200k sampled ranks sit at the oracle's closed-form
-slot, every hit's answer holds the query key and every miss answers n.
+slot, every hit's answer holds the query key and every miss answers n.  Each
+layout then goes back to the sorted keys through the inverse permutation.
 """
 import numpy as np
 import pytest
@@ -64,6 +65,8 @@ def test_config1_bst_u32_whole_layout(dev):
     tl_amd.query_bst(A, Q, ans, kind="u32")
     exp_ans = ol.search("bst", exp, q, "u32", 1)
     assert np.array_equal(ans.cpu().numpy().view(np.uint64), exp_ans)
+    tl_amd.unpermute("bst", A, kind="u32")
+    assert np.array_equal(A.cpu().numpy().view(np.uint32), keys)
 
 
 def _gap_keys(n, seed, kind, dev):
@@ -96,6 +99,8 @@ def test_config2_btree_1e8(dev):
     ans = torch.empty(nq, dtype=torch.int64, device=dev)
     tl_amd.query_btree(A, b, Q, ans, kind="u64")
     _check_answers(A, Q, ans, n, is_hit)
+    tl_amd.unpermute("btree", A, b=b, kind="u64")
+    assert torch.equal(A, keys)
 
 
 @pytest.mark.parametrize("n", [(1 << 29) - 1, 3 * 10 ** 8])
@@ -121,6 +126,8 @@ def test_config3_mveb(n, dev):
     ans = torch.empty(Q.numel(), dtype=torch.int64, device=dev)
     tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind="u64")
     _check_answers(A, Q, ans, n, is_hit)
+    tl_amd.unpermute("mveb", A, kind="u64")
+    assert torch.equal(A, keys)
 
 
 def test_config4_btree_2p30_learned(dev):
@@ -158,3 +165,6 @@ def test_config4_btree_2p30_learned(dev):
     tl_amd.query_btree_learned(A, b, model, Q, lrn, stats, kind="u64")
     assert torch.equal(plain, lrn)
     assert int(stats.item()) == 0   # hits never fall back
+    del plain, lrn, Q
+    tl_amd.unpermute("btree", A, b=b, kind="u64")
+    assert torch.equal(A, keys)     # the inverse brings the layout back to sorted order

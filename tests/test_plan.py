@@ -59,3 +59,102 @@ def test_full_size_plans_are_in_bounds():
         for kind in ("u32", "u64"):
             buf = tl_amd.plan_export(layout, n, b, kind)
             assert int(buf[0]) > 0
+
+
+@pytest.mark.parametrize("layout,bs", [("btree", (16, 8, 2, 3, 32, 1)), ("bst", (1,)),
+                                       ("mveb", (0,))])
+@pytest.mark.parametrize("lds", [256, 2048])
+@pytest.mark.parametrize("row_stash", [None, "0", "40"])
+def test_inverse_plans_replay_to_identity(layout, bs, lds, row_stash, monkeypatch):
+    """The inverse plan takes every layout back to sorted order, and the
+    forward then inverse plan leaves arbitrary data unchanged."""
+    if row_stash is not None:
+        monkeypatch.setenv("TL_ROW_STASH_MAX", row_stash)
+    rng = np.random.default_rng(31)
+    sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 5000, 17 ** 3 - 1, 17 ** 3,
+             33333, 65535, 65536]
+    for b in bs:
+        for n in sizes:
+            inv = tl_amd.plan_export(layout, n, b, "u64", lds, inverse=True)
+            A = ol.layout_ranks(layout, n, b).astype(np.int64)
+            pe.replay(inv, A)
+            assert np.array_equal(A, np.arange(n)), (layout, b, lds, n)
+            X = rng.integers(-(1 << 40), 1 << 40, size=n)
+            Y = X.copy()
+            pe.replay(tl_amd.plan_export(layout, n, b, "u64", lds), Y)
+            pe.replay(inv, Y)
+            assert np.array_equal(X, Y), (layout, b, lds, n)
+
+
+class _Move:
+    def __init__(self, op):
+        (self.kind, self.P, self.L, self.k, self.r, self.l, self.CH,
+         self.nchunk) = (int(v) for v in op[11:19])
+        self.nmoves = int(op[3])
+
+
+def _written(m, dst, ln):
+    """exact key intervals a move stores into the array"""
+    if m.kind == pe.MV_ROW_SPREAD:   # keys and the separators from the stash
+        return [(dst, dst + ln // m.l * (m.l + 1))]
+    if m.kind == pe.MV_ROW_SLIDE:
+        return [(dst, dst + ln // (m.l + 1) * m.l)]
+    return [(dst, dst + ln)]
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_chained_moves_wait_for_every_overwritten_source(inverse):
+    """Every chained move of forward and inverse plans: a move that stores over
+    the source of another move has the larger number and waits for it (the
+    device's wait_deps, mirrored in the batch emulator), and replaying the
+    moves one after another in number order gives the op's specified effect."""
+    import bisect
+    import batch_emulator as bem
+    cases = [("btree", 16, 10 ** 6), ("btree", 16, 17 ** 4 + 12345), ("btree", 2, 300001),
+             ("bst", 1, 1 << 21), ("bst", 1, 3 << 19), ("mveb", 0, (1 << 24) - 1),
+             ("mveb", 0, (1 << 24) + 12345), ("mveb", 0, 5 * 10 ** 6)]
+    kinds_seen = set()
+    for layout, b, n in cases:
+        buf = tl_amd.plan_export(layout, n, b, "u64", 0, inverse=inverse)
+        ops, blob = pe.decode(buf)
+        for op in ops:
+            if int(op[0]) != pe.OP_CHAINED:
+                continue
+            m = _Move(op)
+            kinds_seen.add(m.kind)
+            moves = [bem.move_at(m, t) for t in range(m.nmoves)]
+            srcs = sorted((s, s + ln, t) for t, (s, d, ln) in enumerate(moves))
+            starts = [x[0] for x in srcs]
+            for t, (s, d, ln) in enumerate(moves):
+                waited = set(bem.awaited_moves(m, t, d, ln))
+                for a, z in _written(m, d, ln):
+                    i = max(0, bisect.bisect_right(starts, a) - 1)
+                    while i < len(srcs) and srcs[i][0] < z:
+                        s0, s1, u = srcs[i]
+                        if s1 > a and u != t:
+                            assert u < t and u in waited, (layout, n, m.kind, t, u)
+                        i += 1
+            # sequential replay in move order against the specified effect
+            one = np.concatenate([buf[:2], op.astype(np.uint64)]).astype(np.uint64)
+            one[0], one[1] = 1, 0
+            sep = -1 - np.arange(m.r, dtype=np.int64)   # stash contents for a row spread
+            spec = np.arange(n, dtype=np.int64)
+            pe.replay(one, spec, stash=sep.copy())
+            seq = np.arange(n, dtype=np.int64)
+            for s, d, ln in moves:
+                v = seq[s:s + ln].copy()
+                if m.kind == pe.MV_ROW_SLIDE:
+                    rows = v.reshape(-1, m.l + 1)[:, :m.l].reshape(-1)
+                    seq[d:d + rows.size] = rows
+                elif m.kind == pe.MV_ROW_SPREAD:
+                    rows = seq[d:d + ln // m.l * (m.l + 1)].reshape(-1, m.l + 1)
+                    rows[:, :m.l] = v.reshape(-1, m.l)
+                    j0 = (s - m.P - m.r) // m.l
+                    rows[:, m.l] = sep[j0:j0 + ln // m.l]
+                else:
+                    seq[d:d + ln] = v
+            assert np.array_equal(seq, spec), (layout, n, m.kind)
+    if inverse:
+        assert {pe.MV_ROW_SPREAD, pe.MV_SPARSE_SPREAD, pe.MV_SHIFT_LEFT} <= kinds_seen
+    else:
+        assert {pe.MV_ROW_SLIDE, pe.MV_SPARSE_SLIDE, pe.MV_SHIFT_RIGHT} <= kinds_seen

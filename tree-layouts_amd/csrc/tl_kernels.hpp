@@ -111,7 +111,20 @@ __global__ __launch_bounds__(256) void k_bitrev_levels(T *__restrict__ A, uint32
 //   MV_ROW_SLIDE: r rows of (l keys, 1 separator) at P become
 //     [r separators (left in the stash)][r*l keys]; a move carries CH whole
 //     rows, writes their keys contiguously and their separators to the stash.
-enum : uint32_t { MV_SHIFT_RIGHT = 0, MV_SHIFT_LEFT = 1, MV_SPARSE_SLIDE = 2, MV_ROW_SLIDE = 3 };
+//   MV_ROW_SPREAD, MV_SPARSE_SPREAD: the inverses (layout -> sorted order).
+//     The keys move left, back into rows of l keys with one gap after each
+//     row; moves are numbered from the left, so every source a move
+//     overwrites belongs to a move with a smaller number.  A row spread also
+//     fills each gap with the row's separator from the stash (stash[j] for
+//     row j); the roots of a sparse spread are put back by strided copies.
+enum : uint32_t {
+    MV_SHIFT_RIGHT = 0,
+    MV_SHIFT_LEFT = 1,
+    MV_SPARSE_SLIDE = 2,
+    MV_ROW_SLIDE = 3,
+    MV_ROW_SPREAD = 4,
+    MV_SPARSE_SPREAD = 5
+};
 
 // keys per thread of a chained move (256 threads per move)
 constexpr int CHAINED_PER = 32;
@@ -161,6 +174,34 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
         len = min(m.CH, m.l - c * m.CH);
         dst = m.P + m.r + m.k + i * m.l + c * m.CH;
     }
+}
+
+// moves of the inverse kinds (MV_ROW_SPREAD, MV_SPARSE_SPREAD), numbered
+// from the left
+__device__ inline void move_at_spread(const MoveSpec &m, uint64_t t, uint64_t &src, uint64_t &dst,
+                                      uint64_t &len) {
+    if (m.kind == MV_ROW_SPREAD) {
+        // rows j0 .. j1-1: keys [P + r + j0*l, +len) -> rows at P + j*(l+1)
+        const uint64_t j0 = t * m.CH;
+        const uint64_t j1 = min(m.r, j0 + m.CH);
+        src = m.P + m.r + j0 * m.l;
+        len = (j1 - j0) * m.l;
+        dst = m.P + j0 * (m.l + 1);
+        return;
+    }
+    // leaf blocks first (block 0, chunk 0 first), then the tail chunks
+    const uint64_t nleaf = m.r * m.nchunk;
+    if (t < nleaf) {
+        const uint64_t i = t / m.nchunk, c = t - i * m.nchunk;
+        src = m.P + m.r + m.k + i * m.l + c * m.CH;
+        len = min(m.CH, m.l - c * m.CH);
+        dst = m.P + i * (m.l + 1) + c * m.CH;
+        return;
+    }
+    const uint64_t c = t - nleaf;
+    dst = m.P + m.r * (m.l + 1) + c * m.CH;
+    len = min(m.CH, m.L - c * m.CH);
+    src = dst + m.k;
 }
 
 // Bounded spin wait.  Polls with relaxed agent-scope loads (no cache
@@ -264,11 +305,83 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
     }
 }
 
-// store of one loaded move: contiguous, or (row slide) keys contiguous and
-// every (l+1)-th element to the stash
+// waits of the inverse kinds: the sources of all moves tile [S0, S1) left
+// to right in move order, so the moves overlapping [dst, dst + span) form one
+// range of numbers
+__device__ inline void wait_deps_spread(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
+                                        uint32_t *flags, uint32_t *err, uint64_t limit) {
+    const uint64_t S0 = m.P + m.r + (m.kind == MV_SPARSE_SPREAD ? m.k : 0);
+    const uint64_t span = m.kind == MV_ROW_SPREAD ? len / m.l * (m.l + 1) : len;
+    const uint64_t e = dst + span;   // written keys lie in [dst, e)
+    if (e <= S0) return;
+    const uint64_t a = (dst > S0 ? dst : S0) - S0, z = e - 1 - S0;
+    uint64_t ulo, uhi;
+    if (m.kind == MV_ROW_SPREAD) {
+        ulo = a / (m.CH * m.l);
+        uhi = z / (m.CH * m.l);
+    } else {
+        const uint64_t leaf = m.r * m.l;
+        auto num = [&](uint64_t x) {
+            if (x < leaf) {
+                const uint64_t i = x / m.l;
+                return i * m.nchunk + (x - i * m.l) / m.CH;
+            }
+            return m.r * m.nchunk + (x - leaf) / m.CH;
+        };
+        ulo = num(a);
+        uhi = num(z);
+    }
+    for (uint64_t u = ulo; u <= uhi && u < t; u++)
+        if (!wait_flag(&flags[u], err, limit)) return;
+}
+
+// loads of a row-spread move, indexed by the destination, which is rows of
+// (l keys, 1 separator): key c of row j comes from src + j*l + c and the
+// separator from stash[j0 + j], so that both the loads and the stores of
+// every instruction cover contiguous addresses.  The inverse kinds have a
+// kernel instantiation each (MODE), which keeps every kernel on one straight
+// path and its registers low.
 template <typename T, int PER>
+__device__ inline void load_row_spread(const T *A, const T *stash, const MoveSpec &ms,
+                                       uint64_t src, uint64_t len, T (&v)[PER]) {
+    const uint32_t l = (uint32_t)ms.l, L1 = l + 1;
+    const uint32_t tot = (uint32_t)(len / l) * L1;
+    const T *st = stash + (src - ms.P - ms.r) / l;
+    // (row, column) of destination index threadIdx.x + j*256, stepped
+    uint32_t row = threadIdx.x / L1, c = threadIdx.x - row * L1;
+    const uint32_t rq = 256u / L1, cq = 256u - rq * L1;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t idx = threadIdx.x + (uint32_t)j * 256u;
+        if (idx < tot) v[j] = c == l ? st[row] : A[src + (uint64_t)row * l + c];
+        row += rq;
+        c += cq;
+        if (c >= L1) {
+            c -= L1;
+            row++;
+        }
+    }
+}
+
+// store of one loaded move: contiguous (a row spread's v is already in
+// destination order), or (row slide) keys contiguous and every (l+1)-th
+// element to the stash
+// kernel modes: the forward kinds (shifts, slides), a row spread, a sparse
+// spread
+enum : int { CH_FORWARD = 0, CH_ROW_SPREAD = 1, CH_SPARSE_SPREAD = 2 };
+
+template <typename T, int PER, int MODE = CH_FORWARD>
 __device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t src, uint64_t dst,
                                   uint64_t len, const T (&v)[PER]) {
+    if constexpr (MODE == CH_ROW_SPREAD) {
+        const uint64_t tot = len / ms.l * (ms.l + 1);
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+            if (idx < tot) A[dst + idx] = v[j];
+        }
+        return;
+    }
     if (ms.kind == MV_ROW_SLIDE) {
         const uint32_t L1 = (uint32_t)(ms.l + 1), l = (uint32_t)ms.l;
         T *st = stash + (src - ms.P) / L1;
@@ -292,7 +405,7 @@ __device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t s
     }
 }
 
-template <typename T, int PER>
+template <typename T, int PER, int MODE = CH_FORWARD>
 __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
                                                  uint32_t *flags, unsigned long long *counter,
                                                  uint32_t *err, uint64_t limit, T *stash) {
@@ -304,22 +417,32 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
         __syncthreads();
         if (t >= nmoves) return;
         uint64_t src, dst, len;
-        move_at(ms, t, src, dst, len);
+        if constexpr (MODE != CH_FORWARD)
+            move_at_spread(ms, t, src, dst, len);
+        else
+            move_at(ms, t, src, dst, len);
         T v[PER];
+        if constexpr (MODE == CH_ROW_SPREAD) {
+            load_row_spread<T, PER>(A, stash, ms, src, len, v);
+        } else {
 #pragma unroll
-        for (int j = 0; j < PER; j++) {
-            const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
-            if (idx < len) v[j] = A[src + idx];
+            for (int j = 0; j < PER; j++) {
+                const uint64_t idx = threadIdx.x + (uint64_t)j * blockDim.x;
+                if (idx < len) v[j] = A[src + idx];
+            }
         }
         // every wave's loads complete before the chunk is published as read
         loads_done();
         __syncthreads();
         if (threadIdx.x == 0) {
             __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            wait_deps(ms, t, dst, len, flags, err, limit);
+            if constexpr (MODE != CH_FORWARD)
+                wait_deps_spread(ms, t, dst, len, flags, err, limit);
+            else
+                wait_deps(ms, t, dst, len, flags, err, limit);
         }
         __syncthreads();
-        store_move<T, PER>(A, stash, ms, src, dst, len, v);
+        store_move<T, PER, MODE>(A, stash, ms, src, dst, len, v);
     }
 }
 

@@ -51,11 +51,13 @@ __device__ inline uint32_t lds_slot(uint32_t x, uint32_t n, uint32_t kind, uint3
     return ((1u << t) | rev_low_bits32(y & ((1u << t) - 1), t)) - 1;
 }
 
+// inv = 1 applies the inverse permutation (slot -> rank instead of rank ->
+// slot): the keys go into LDS in place and come out gathered through the slots.
 template <typename T>
 __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                                   uint32_t nruns, uint64_t nitems, uint32_t per,
                                                   uint32_t n, uint32_t kind, uint32_t b,
-                                                  uint32_t rows, uint32_t magic) {
+                                                  uint32_t rows, uint32_t magic, uint32_t inv) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *s = reinterpret_cast<T *>(smem_raw);
     uint32_t *tbl = reinterpret_cast<uint32_t *>(smem_raw + (size_t)per * n * sizeof(T));
@@ -87,7 +89,8 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
         for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * blockDim.x;
             if (i < tot) {
-                const uint32_t p = table ? tbl[ii[u]] : lds_slot(ii[u], n, kind, b, rows, sh);
+                const uint32_t p =
+                    inv ? ii[u] : (table ? tbl[ii[u]] : lds_slot(ii[u], n, kind, b, rows, sh));
                 s[it[u] * n + p] = v[u];
             }
         }
@@ -95,7 +98,8 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
         const uint32_t t = table ? __umulhi(i, magic) : 0, x = i - t * n;
-        A[s_off[t] + x] = s[i];
+        const uint32_t p = inv ? (table ? tbl[x] : lds_slot(x, n, kind, b, rows, sh)) : x;
+        A[s_off[t] + x] = s[t * n + p];
     }
 }
 
@@ -134,12 +138,14 @@ __global__ __launch_bounds__(256) void k_reverse_runs(T *__restrict__ A, const u
     }
 }
 
+// unit u[k] of a cycle moves to u[k+1] (the last to u[0]); inv = 1 moves
+// them the other way round
 template <typename T>
 __global__ __launch_bounds__(256) void k_unit_merge_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                                          uint32_t nruns, uint64_t nparents, uint64_t unit,
                                                          const uint64_t *__restrict__ cstart,
                                                          const uint64_t *__restrict__ cunits,
-                                                         uint32_t nchunks) {
+                                                         uint32_t nchunks, uint32_t inv) {
     const uint32_t chunk = blockIdx.x % nchunks;
     const uint32_t cyc = blockIdx.x / nchunks;
     const uint64_t c = (uint64_t)chunk * blockDim.x + threadIdx.x;
@@ -149,6 +155,23 @@ __global__ __launch_bounds__(256) void k_unit_merge_runs(T *__restrict__ A, cons
     const uint64_t *u = cunits + s0;
     for (uint64_t p = blockIdx.y; p < nparents; p += gridDim.y) {
         T *a = A + run_offset(runs, nruns, p) + c;
+        if (inv) {
+            T hold = a[u[0] * unit];
+            int64_t k = 0;
+            while (k < L - 1) {
+                T v[8];
+                const int nb = L - 1 - k < 8 ? (int)(L - 1 - k) : 8;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (j < nb) v[j] = a[u[k + j + 1] * unit];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (j < nb) a[u[k + j] * unit] = v[j];
+                k += nb;
+            }
+            a[u[L - 1] * unit] = hold;
+            continue;
+        }
         T hold = a[u[L - 1] * unit];
         int64_t k = L - 2;
         while (k >= 0) {
@@ -166,9 +189,16 @@ __global__ __launch_bounds__(256) void k_unit_merge_runs(T *__restrict__ A, cons
     }
 }
 
+// one workgroup per partition of r rows of (l leaf keys, 1 root): the roots
+// collect in LDS while the leaf keys stream right from the end, then the
+// roots are written to the front.  inv = 1 undoes it: the front r roots go to
+// LDS, then the leaf keys stream left from the front of the leaf area, each
+// batch read before any of it is written, and each root lands after its
+// row's last key.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
-                                                            uint32_t nruns, uint64_t r, uint64_t l) {
+                                                            uint32_t nruns, uint64_t r, uint64_t l,
+                                                            uint32_t inv) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *roots = reinterpret_cast<T *>(smem_raw);
     T *a = A + run_offset(runs, nruns, blockIdx.x);
@@ -176,6 +206,35 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
     const uint64_t total = r * L1;
     constexpr int U = 8;
     const uint64_t batch = (uint64_t)blockDim.x * U;
+    if (inv) {
+        // leaf keys stream left in source order: key y of the leaf area
+        // (row y / l, column y % l) lands at row*(l+1) + column, never to the
+        // right of where it was read; a row's root follows its last key
+        for (uint64_t i = threadIdx.x; i < r; i += blockDim.x) roots[i] = a[i];
+        __syncthreads();
+        const uint64_t nleaf = r * l;
+        for (uint64_t lo = 0; lo < nleaf; lo += batch) {
+            T v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint64_t y = lo + threadIdx.x + (uint64_t)k * blockDim.x;
+                if (y < nleaf) v[k] = a[r + y];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint64_t y = lo + threadIdx.x + (uint64_t)k * blockDim.x;
+                if (y < nleaf) {
+                    const uint64_t row =
+                        nleaf < (1ull << 32) ? (uint64_t)((uint32_t)y / (uint32_t)l) : y / l;
+                    const uint64_t cc = y - row * l, d = row * L1 + cc;
+                    a[d] = v[k];
+                    if (cc == l - 1) a[d + 1] = roots[row];
+                }
+            }
+        }
+        return;
+    }
     uint64_t hi = total;
     while (hi > 0) {
         const uint64_t lo = hi > batch ? hi - batch : 0;
@@ -231,7 +290,8 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             const uint32_t magic = (uint32_t)((0xFFFFFFFFull + n) / n);
             hipLaunchKernelGGL(k_lds_runs<T>, dim3((uint32_t)groups), dim3(threads), bytes, s, A,
                                runs, (uint32_t)op.nruns, op.nitems, (uint32_t)per, (uint32_t)n,
-                               (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3], magic);
+                               (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3], magic,
+                               (uint32_t)op.p[5]);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -279,14 +339,15 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             const uint32_t gy = (uint32_t)std::min<uint64_t>(op.nitems, 65535);
             hipLaunchKernelGGL(k_unit_merge_runs<T>, dim3((uint32_t)(nchunks * ncyc), gy), dim3(256),
                                0, s, A, runs, (uint32_t)op.nruns, op.nitems, unit, blob + op.p[1],
-                               blob + op.p[3], nchunks);
+                               blob + op.p[3], nchunks, (uint32_t)op.p[4]);
             TL_LAUNCH_CHECK();
             break;
         }
         case OP_SPARSE_STREAM: {
             const uint64_t r = op.p[0], l = op.p[1];
             hipLaunchKernelGGL(k_sparse_stream_runs<T>, dim3((uint32_t)op.nitems), dim3(256),
-                               r * sizeof(T), s, A, runs, (uint32_t)op.nruns, r, l);
+                               r * sizeof(T), s, A, runs, (uint32_t)op.nruns, r, l,
+                               (uint32_t)op.p[2]);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -308,8 +369,18 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             if (!preclear) TL_HIP(hipMemsetAsync(op_flags, 0, (words + 16) * 4, s));
             unsigned long long *counter = (unsigned long long *)(op_flags + words);
             const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
-            hipLaunchKernelGGL((k_chained<T, CHAINED_PER>), dim3(grid), dim3(256), 0, s, A, op.ms, nmoves,
-                               op_flags, counter, c->err_flag, c->wait_ticks, stash);
+            if (op.ms.kind == MV_ROW_SPREAD)
+                hipLaunchKernelGGL((k_chained<T, CHAINED_PER, CH_ROW_SPREAD>), dim3(grid), dim3(256),
+                                   0, s, A, op.ms, nmoves, op_flags, counter, c->err_flag,
+                                   c->wait_ticks, stash);
+            else if (op.ms.kind == MV_SPARSE_SPREAD)
+                hipLaunchKernelGGL((k_chained<T, CHAINED_PER, CH_SPARSE_SPREAD>), dim3(grid),
+                                   dim3(256), 0, s, A, op.ms, nmoves, op_flags, counter,
+                                   c->err_flag, c->wait_ticks, stash);
+            else
+                hipLaunchKernelGGL((k_chained<T, CHAINED_PER, CH_FORWARD>), dim3(grid), dim3(256),
+                                   0, s, A, op.ms, nmoves, op_flags, counter, c->err_flag,
+                                   c->wait_ticks, stash);
             TL_LAUNCH_CHECK();
             if (preclear) op_flags += words + 16;
             break;
@@ -412,7 +483,7 @@ static tl_status check_args(Layout lay, const void *dev_A, uint64_t n, uint64_t 
 }
 
 static tl_status permute(Layout lay, void *dev_A, uint64_t n, uint64_t b, tl_dtype dt,
-                         void *stream, float *ms) {
+                         void *stream, float *ms, bool inverse = false) {
     uint32_t W;
     TL_TRY(check_args(lay, dev_A, n, b, dt, W));
     Ctx *c;
@@ -420,6 +491,14 @@ static tl_status permute(Layout lay, void *dev_A, uint64_t n, uint64_t b, tl_dty
     hipStream_t s = (hipStream_t)stream;
     Plan pl;
     build_plan(pl, tuning_for(c), lay, n, b, W);
+    if (inverse) {
+        Plan fwd;
+        fwd.ops.swap(pl.ops);
+        fwd.blob.swap(pl.blob);
+        fwd.max_stash = pl.max_stash;
+        fwd.max_moves = pl.max_moves;
+        invert_plan(fwd, pl);
+    }
     std::string why;
     if (!validate_plan(pl, n, why)) {
         set_error(why);
@@ -442,7 +521,7 @@ static tl_status permute(Layout lay, void *dev_A, uint64_t n, uint64_t b, tl_dty
 //   [type, nruns, nitems, p0..p7, ms.kind, ms.P, ms.L, ms.k, ms.r, ms.l, ms.CH, ms.nchunk,
 //    runs_at, 0...] followed by the blob.
 static tl_status export_plan(Layout lay, uint64_t n, uint64_t b, tl_dtype dt, uint64_t lds_bytes,
-                             uint64_t *buf, uint64_t cap, uint64_t *used) {
+                             uint64_t *buf, uint64_t cap, uint64_t *used, bool inverse) {
     uint32_t W;
     TL_TRY(check_args(lay, (void *)1, n, b, dt, W));
     Tuning tu;
@@ -450,6 +529,10 @@ static tl_status export_plan(Layout lay, uint64_t n, uint64_t b, tl_dtype dt, ui
     if (lds_bytes) tu.lds_bytes = lds_bytes;
     Plan pl;
     build_plan(pl, tu, lay, n, b, W);
+    if (inverse) {
+        Plan fwd = pl;
+        invert_plan(fwd, pl);
+    }
     std::string why;
     if (!validate_plan(pl, n, why)) {
         set_error(why);
@@ -499,12 +582,29 @@ extern "C" tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, vo
     return tl::permute(tl::L_MVEB, dev_A, n, 0, dtype, stream, ms);
 }
 
+extern "C" tl_status tl_unpermute_bst(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                      float *ms) {
+    return tl::permute(tl::L_BST, dev_A, n, 1, dtype, stream, ms, true);
+}
+
+extern "C" tl_status tl_unpermute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                        void *stream, float *ms) {
+    return tl::permute(tl::L_BTREE, dev_A, n, b, dtype, stream, ms, true);
+}
+
+extern "C" tl_status tl_unpermute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                       float *ms) {
+    return tl::permute(tl::L_MVEB, dev_A, n, 0, dtype, stream, ms, true);
+}
+
 extern "C" tl_status tl_plan_export(uint32_t layout, uint64_t n, uint64_t b, tl_dtype dtype,
                                     uint64_t lds_bytes, uint64_t *buf, uint64_t cap,
                                     uint64_t *used) {
+    const bool inverse = (layout & tl::PLAN_INVERSE) != 0;
+    layout &= ~tl::PLAN_INVERSE;
     if (layout > 2) {
-        tl::set_error("layout must be 0 (BST), 1 (B-tree) or 2 (mvEB)");
+        tl::set_error("layout must be 0 (BST), 1 (B-tree) or 2 (mvEB), optionally | 0x100");
         return TL_EINVAL;
     }
-    return tl::export_plan((tl::Layout)layout, n, b, dtype, lds_bytes, buf, cap, used);
+    return tl::export_plan((tl::Layout)layout, n, b, dtype, lds_bytes, buf, cap, used, inverse);
 }

@@ -614,11 +614,18 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
                 if (m.CH == 0 || m.CH > CHAINED_KEYS || m.CH > m.l ||
                     op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
                     neg = true;
-            } else if (m.kind == MV_ROW_SLIDE) {
+            } else if (m.kind == MV_SPARSE_SPREAD) {
+                end = m.P + m.r * (m.l + 1) + m.L + m.k;
+                if (m.CH == 0 || m.CH > CHAINED_KEYS || m.CH > m.l || m.nchunk * m.CH < m.l ||
+                    op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
+                    neg = true;
+                break;
+            } else if (m.kind == MV_ROW_SLIDE || m.kind == MV_ROW_SPREAD) {
                 end = m.P + m.r * (m.l + 1);
                 if (m.CH == 0 || m.CH * (m.l + 1) > CHAINED_KEYS || m.nchunk * m.CH < m.r ||
-                    m.r > pl.max_stash || op.p[0] != m.nchunk)
+                    op.p[0] != m.nchunk || m.l == 0)
                     neg = true;
+                if (m.r > pl.max_stash) neg = true;
                 break;
             } else if (m.kind == MV_SHIFT_RIGHT) {
                 end = m.P + m.L + m.k;
@@ -651,6 +658,53 @@ inline bool width_of(tl_dtype dt, uint32_t &W) {
 }
 
 enum Layout { L_BST, L_BTREE, L_MVEB };
+
+// flag in a layout number: the inverse permutation (layout -> sorted order)
+constexpr uint32_t PLAN_INVERSE = 0x100;
+
+// The inverse of a forward plan (layout -> sorted order): the launches in
+// reverse order, each replaced by its inverse.  Reversals and bit-reversal
+// tiles are involutions; LDS layouts, unit merges and streamed root gathers
+// run with their inverse flag; copies swap direction; a shift becomes the
+// opposite shift of the moved region; a row slide becomes a row spread (which
+// takes the separators back from the stash the inverted copy before it
+// fills); a sparse slide becomes a sparse spread (its roots are put back by
+// the inverted copies around it).
+// Runs, cycle tables, stash and flag sizes are the forward plan's.
+inline void invert_plan(const Plan &f, Plan &g) {
+    g.ops.clear();
+    g.blob = f.blob;
+    g.max_stash = f.max_stash;
+    g.max_moves = f.max_moves;
+    for (size_t i = f.ops.size(); i-- > 0;) {
+        Op op = f.ops[i];
+        switch (op.type) {
+        case OP_LDS: op.p[5] ^= 1; break;
+        case OP_UNIT_MERGE: op.p[4] ^= 1; break;
+        case OP_SPARSE_STREAM: op.p[2] ^= 1; break;
+        case OP_COPY: op.p[3] ^= 1; break;
+        case OP_REVERSE:
+        case OP_BITREV:
+        case OP_BITREV_LEVELS: break;
+        case OP_CHAINED: {
+            MoveSpec &m = op.ms;
+            if (m.kind == MV_SHIFT_RIGHT) {
+                m.kind = MV_SHIFT_LEFT;
+                m.P += m.k;
+            } else if (m.kind == MV_SHIFT_LEFT) {
+                m.kind = MV_SHIFT_RIGHT;
+                m.P -= m.k;
+            } else if (m.kind == MV_SPARSE_SLIDE) {
+                m.kind = MV_SPARSE_SPREAD;
+            } else if (m.kind == MV_ROW_SLIDE) {
+                m.kind = MV_ROW_SPREAD;
+            }
+            break;
+        }
+        }
+        g.ops.push_back(op);
+    }
+}
 
 inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint64_t b, uint32_t W) {
     // scratch stays at most n/16 keys (or 2^24 keys for small arrays)

@@ -43,6 +43,12 @@ _SIGS = {
                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
     "tl_permute_mveb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                         ctypes.POINTER(ctypes.c_float)],
+    "tl_unpermute_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                         ctypes.POINTER(ctypes.c_float)],
+    "tl_unpermute_btree": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_unpermute_mveb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                          ctypes.POINTER(ctypes.c_float)],
     "tl_query_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                      ctypes.POINTER(ctypes.c_float)],
@@ -190,6 +196,20 @@ def permute(layout, A, b=16, kind=None, stream=None, timed=True):
     return permute_mveb(A, kind, stream, timed)
 
 
+def unpermute(layout, A, b=16, kind=None, stream=None, timed=True):
+    """In-place inverse of permute(layout, ...): layout order -> sorted order."""
+    k = _kind(A, kind)
+    L = lib()
+    if layout == "bst":
+        return _timed_call(L.tl_unpermute_bst, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                           timed=timed)
+    if layout == "btree":
+        return _timed_call(L.tl_unpermute_btree, _ptr(A), A.numel(), b, DTYPE[k],
+                           _stream(stream), timed=timed)
+    return _timed_call(L.tl_unpermute_mveb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                       timed=timed)
+
+
 # ---- queries ---------------------------------------------------------------
 
 def query_bst(A, Q, ans, kind=None, stream=None, timed=True):
@@ -244,15 +264,16 @@ def veb_build_table(n, d, root_depth=0):
     return [(e.L, e.R, e.D) for e in arr]
 
 
-def plan_export(layout, n, b=1, kind="u64", lds_bytes=0):
-    """Serialized dry-run plan of a permutation (list of uint64 words)."""
+def plan_export(layout, n, b=1, kind="u64", lds_bytes=0, inverse=False):
+    """Serialized dry-run plan of a permutation, or of its inverse (list of
+    uint64 words)."""
     import numpy as np
     used = ctypes.c_uint64(0)
     L = lib()
-    _check(L.tl_plan_export(LAYOUT[layout], n, b, DTYPE[kind], lds_bytes, None, 0,
-                            ctypes.byref(used)))
+    lay = LAYOUT[layout] | (0x100 if inverse else 0)
+    _check(L.tl_plan_export(lay, n, b, DTYPE[kind], lds_bytes, None, 0, ctypes.byref(used)))
     buf = np.zeros(used.value, dtype=np.uint64)
-    _check(L.tl_plan_export(LAYOUT[layout], n, b, DTYPE[kind], lds_bytes,
+    _check(L.tl_plan_export(lay, n, b, DTYPE[kind], lds_bytes,
                             buf.ctypes.data_as(ctypes.c_void_p), used.value,
                             ctypes.byref(used)))
     return buf
