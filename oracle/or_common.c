@@ -44,6 +44,16 @@ void or_mveb_split(uint32_t d, uint32_t *root_d, uint32_t *leaf_d) {
     *leaf_d = l;
 }
 
+/* The classic split of the CPU vEB: top tree of floor(d/2) levels, bottom
+ * trees of ceil(d/2) (cpu/src/veb-cycles.h:36-50,114-115,
+ * cpu/src/query-veb.cpp:28-29). */
+void or_classic_split(uint32_t d, uint32_t *root_d, uint32_t *leaf_d) {
+    *root_d = (d - 2) / 2 + 1;
+    *leaf_d = d - *root_d;
+}
+
+typedef void (*split_fn)(uint32_t, uint32_t *, uint32_t *);
+
 /* ------------------------------------------------------------------------ */
 /* In-order DFS fill of the heap-indexed node tree.  Node u owns the slots
  * [u*b, min(u*b+b, n)); its children are nodes u*(b+1)+1+j, j = 0..b.  The
@@ -113,12 +123,12 @@ uint64_t or_btree_pos(uint64_t x, uint64_t n, uint64_t b) {
  * of inc_n keys laid out recursively as its own tree; the top keys whose
  * bottom trees are empty are the largest ranks
  * (gpu/src/mveb-cycles.cuh:299-414). */
-uint64_t or_mveb_pos(uint64_t x, uint64_t n) {
+static uint64_t veb_pos(uint64_t x, uint64_t n, split_fn split) {
     uint64_t base = 0;
     for (;;) {
         if (n <= 1) return base;
         uint32_t d = floor_log2(n) + 1, rd, ld;
-        or_mveb_split(d, &rd, &ld);
+        split(d, &rd, &ld);
         uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
         uint64_t full = (d >= 64) ? ~0ull : (1ull << d) - 1;
         uint64_t nf, inc;
@@ -154,8 +164,19 @@ uint64_t or_mveb_pos(uint64_t x, uint64_t n) {
     }
 }
 
+uint64_t or_mveb_pos(uint64_t x, uint64_t n) { return veb_pos(x, n, or_mveb_split); }
+
+/* the same structure with the classic split: the CPU vEB layout
+ * (cpu/src/veb-cycles.h:34-160) */
+uint64_t or_classic_pos(uint64_t x, uint64_t n) { return veb_pos(x, n, or_classic_split); }
+
 int or_mveb_layout(uint64_t n, uint64_t *pos_of_rank) {
     for (uint64_t x = 0; x < n; x++) pos_of_rank[x] = or_mveb_pos(x, n);
+    return 0;
+}
+
+int or_classic_layout(uint64_t n, uint64_t *pos_of_rank) {
+    for (uint64_t x = 0; x < n; x++) pos_of_rank[x] = or_classic_pos(x, n);
     return 0;
 }
 
@@ -164,33 +185,43 @@ int or_mveb_layout(uint64_t n, uint64_t *pos_of_rank) {
  * (gpu/src/query-mveb.cu:25-47).  Entry t describes the bottom trees that
  * start at depth t: their size L, the size R of the top tree above them and
  * the depth D of that top tree's root. */
-void or_build_table(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth) {
+static void build_table(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth,
+                        split_fn split) {
     if (n <= 1) return;
     uint32_t rd, ld;
-    or_mveb_split(d, &rd, &ld);
+    split(d, &rd, &ld);
     uint64_t root_n = (1ull << rd) - 1;
     uint64_t leaf_n = (1ull << ld) - 1;
     uint32_t i = d - ld;
     table[i].L = leaf_n;
     table[i].R = root_n;
     table[i].D = root_depth;
-    or_build_table(table, root_n, rd, root_depth);
-    or_build_table(&table[i], leaf_n, ld, i + root_depth);
+    build_table(table, root_n, rd, root_depth, split);
+    build_table(&table[i], leaf_n, ld, i + root_depth, split);
+}
+
+void or_build_table(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth) {
+    build_table(table, n, d, root_depth, or_mveb_split);
+}
+
+/* the classic rule (cpu/src/query-veb.cpp:25-41) */
+void or_build_table_classic(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth) {
+    build_table(table, n, d, root_depth, or_classic_split);
 }
 
 /* Table set for a non-perfect mvEB: one table per nested incomplete bottom
  * tree.  idx[j] is where structure j ends (the start of the incomplete tree
  * it contains), the last entry is n; table_idx[j] is the offset of table j in
  * the packed table array (gpu/test/query-mveb-fixedN.cu:42-165). */
-int or_veb_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
-                    or_veb_table *tables, uint64_t *idx, uint32_t *table_idx) {
+static int tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
+                    or_veb_table *tables, uint64_t *idx, uint32_t *table_idx, split_fn split) {
     if (n == 0) return -1;
     uint32_t d = floor_log2(n) + 1;
     *d_out = d;
     uint64_t full = (1ull << d) - 1;
     if (n == full) {
         memset(tables, 0, sizeof(or_veb_table) * d);
-        or_build_table(tables, n, d, 0);
+        build_table(tables, n, d, 0, split);
         *num_tables = 1;
         *table_size = d;
         idx[0] = n;
@@ -207,7 +238,7 @@ int or_veb_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t 
     uint32_t td = d;
     while (td >= 2) {
         uint32_t rd, ld;
-        or_mveb_split(td, &rd, &ld);
+        split(td, &rd, &ld);
         uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
         uint64_t nf = (tn - r) / l;
         uint64_t inc = tn - r - nf * l;
@@ -225,13 +256,25 @@ int or_veb_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t 
     uint32_t off = 0;
     for (uint32_t j = 0; j < nt; j++) {
         table_idx[j] = off;
-        or_build_table(&tables[off], sizes[j], depths[j], 0);
+        build_table(&tables[off], sizes[j], depths[j], 0, split);
         off += depths[j];
         idx[j] = (j + 1 < nt) ? n - sizes[j + 1] : n;
     }
     *num_tables = nt;
     *table_size = tsize;
     return 0;
+}
+
+int or_veb_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
+                    or_veb_table *tables, uint64_t *idx, uint32_t *table_idx) {
+    return tableset(n, d_out, num_tables, table_size, tables, idx, table_idx, or_mveb_split);
+}
+
+/* the classic vEB's table set, built the same way with the classic rule
+ * (cpu/test/query-veb-fixedN.cpp:37-158, where it is an array of pointers) */
+int or_classic_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
+                        or_veb_table *tables, uint64_t *idx, uint32_t *table_idx) {
+    return tableset(n, d_out, num_tables, table_size, tables, idx, table_idx, or_classic_split);
 }
 
 /* ------------------------------------------------------------------------ */

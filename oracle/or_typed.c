@@ -86,6 +86,22 @@
         }                                                                                      \
         return n;                                                                              \
     }                                                                                          \
+    /* cpu/src/query-veb.h:103-137: the same search over the classic vEB      \
+     * with the classic table set; running out of depth answers n */          \
+    void or_search_classic_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,              \
+                                 uint64_t *ans, int p) {                                       \
+        if (n == 0) {                                                                          \
+            for (uint64_t i = 0; i < nq; i++) ans[i] = 0;                                      \
+            return;                                                                            \
+        }                                                                                      \
+        or_veb_table tables[64 * 64];                                                          \
+        uint64_t idx[64];                                                                      \
+        uint32_t tidx[64], d, nt, ts;                                                          \
+        or_classic_tableset(n, &d, &nt, &ts, tables, idx, tidx);                               \
+        _Pragma("omp parallel for schedule(guided) num_threads(p) if (p > 1)")                 \
+        for (uint64_t i = 0; i < nq; i++)                                                      \
+            ans[i] = mveb_one_##SUF(A, n, tables, idx, tidx, nt, q[i]);                        \
+    }                                                                                          \
     /* cpu/src/binary-search.h:30-50 on the sorted array: a hit answers the    \
      * key's index, a miss the index of its predecessor (`right` at the end of \
      * the loop).  With no predecessor (q < A[0], or n = 0) the reference's    \

@@ -36,6 +36,7 @@ typedef struct {
 uint64_t or_ipow(uint64_t base, uint32_t e);
 uint32_t or_ilog(uint64_t n, uint64_t base);      /* floor(log_base n), n >= 1 */
 void     or_mveb_split(uint32_t d, uint32_t *root_d, uint32_t *leaf_d);
+void     or_classic_split(uint32_t d, uint32_t *root_d, uint32_t *leaf_d);
 
 /* ---- independent constructions (family 2) ------------------------------ */
 /* pos_of_rank[r] = slot of the r-th smallest key, in-order DFS of the
@@ -47,6 +48,9 @@ uint64_t or_btree_pos(uint64_t rank, uint64_t n, uint64_t b);
  * gpu/src/mveb-cycles.cuh:240-247) */
 uint64_t or_mveb_pos(uint64_t rank, uint64_t n);
 int      or_mveb_layout(uint64_t n, uint64_t *pos_of_rank);
+/* the same recursion with the classic split (the CPU vEB layout) */
+uint64_t or_classic_pos(uint64_t rank, uint64_t n);
+int      or_classic_layout(uint64_t n, uint64_t *pos_of_rank);
 
 /* ---- mvEB query tables ------------------------------------------------- */
 void or_build_table(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth);
@@ -55,6 +59,10 @@ void or_build_table(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_d
  * idx/table_idx >= 64 entries.  For a perfect n it returns num_tables = 1. */
 int or_veb_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
                     or_veb_table *tables, uint64_t *idx, uint32_t *table_idx);
+/* classic vEB: table (cpu/src/query-veb.cpp:25-41) and table set */
+void or_build_table_classic(or_veb_table *table, uint64_t n, uint32_t d, uint32_t root_depth);
+int or_classic_tableset(uint64_t n, uint32_t *d_out, uint32_t *num_tables, uint32_t *table_size,
+                        or_veb_table *tables, uint64_t *idx, uint32_t *table_idx);
 
 /* ---- restated permutations (family 1) ---------------------------------- *
  * p = number of OpenMP threads (1 = the reference's serial path).
@@ -65,6 +73,8 @@ int or_permute_btree_u32(uint32_t *A, uint64_t n, uint64_t b, int p);
 int or_permute_btree_u64(uint64_t *A, uint64_t n, uint64_t b, int p);
 int or_permute_mveb_u32(uint32_t *A, uint64_t n);
 int or_permute_mveb_u64(uint64_t *A, uint64_t n);
+int or_permute_classic_u32(uint32_t *A, uint64_t n, int p);
+int or_permute_classic_u64(uint64_t *A, uint64_t n, int p);
 
 /* ---- restated queries: answers[i] = slot of queries[i] or n ------------ */
 #define OR_QDECL(SUF, T)                                                                    \
@@ -74,6 +84,9 @@ int or_permute_mveb_u64(uint64_t *A, uint64_t n);
                                uint64_t *ans, int p);                                       \
     void or_search_mveb_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,              \
                               uint64_t *ans, int p);                                        \
+    /* classic vEB layout A, tables of the classic rule */                                \
+    void or_search_classic_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,           \
+                                 uint64_t *ans, int p);                                     \
     /* sorted A: hit -> index, miss -> predecessor's index, none -> n */                    \
     void or_search_binary_##SUF(const T *A, uint64_t n, const T *q, uint64_t nq,            \
                                 uint64_t *ans, int p);

@@ -8,7 +8,8 @@ Layouts do not depend on key values, so a rank layout fixes the layout of any
 sorted key set of that size.
 
 Run:  python tests/golden/make_golden.py           (every fixture)
-      python tests/golden/make_golden.py binary    (binary_search.npz only)
+      python tests/golden/make_golden.py binary    (binary_search.npz and
+                                                    veb_classic_digests.json only)
 """
 import json
 import os
@@ -76,11 +77,27 @@ def binary_search_fixtures():
     print("wrote", len(out) // 3, "binary-search cases")
 
 
+def classic_veb_fixtures():
+    """Digests of the reference's CPU classic vEB permutation
+    (cpu/src/veb-cycles.h:381) of 0..n-1, perfect and non-perfect."""
+    out = {}
+    sizes = set(range(1, 4097))
+    for k in range(2, 25):
+        sizes.update({(1 << k) - 2, (1 << k) - 1, 1 << k, (1 << k) + 1})
+    sizes.update({10 ** 6, 3 * 10 ** 6 + 7, 12345678})
+    for n in sorted(sizes):
+        out[f"veb_n{n}"] = ol.digest(ref_layout("veb_classic", n, 0))
+    with open(os.path.join(HERE, "veb_classic_digests.json"), "w") as f:
+        json.dump(out, f)
+    print("wrote", len(out), "classic vEB digests")
+
+
 def main():
     if ol.ref() is None:
         sys.exit("oracle/_ref/libref.so is not built (make -C oracle ref)")
     binary_search_fixtures()
-    if sys.argv[1:] == ["binary"]:
+    classic_veb_fixtures()
+    if sys.argv[1:] in (["binary"], ["veb"]):
         return
     small = {}
     for layout, b in [("bst", 1), ("btree", 2), ("btree", 3), ("btree", 8), ("btree", 16),

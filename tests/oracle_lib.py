@@ -35,6 +35,15 @@ def oracle():
         o.or_btree_pos.argtypes = [C, C, C]; o.or_btree_pos.restype = C
         o.or_mveb_pos.argtypes = [C, C]; o.or_mveb_pos.restype = C
         o.or_mveb_layout.argtypes = [C, VP]; o.or_mveb_layout.restype = I
+        o.or_classic_pos.argtypes = [C, C]; o.or_classic_pos.restype = C
+        o.or_classic_layout.argtypes = [C, VP]; o.or_classic_layout.restype = I
+        o.or_build_table_classic.argtypes = [ctypes.POINTER(VebTable), C, ctypes.c_uint32,
+                                             ctypes.c_uint32]
+        o.or_classic_tableset.argtypes = [C, ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(VebTable),
+                                          ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.POINTER(ctypes.c_uint32)]
         o.or_build_table.argtypes = [ctypes.POINTER(VebTable), C, ctypes.c_uint32, ctypes.c_uint32]
         o.or_veb_tableset.argtypes = [C, ctypes.POINTER(ctypes.c_uint32),
                                       ctypes.POINTER(ctypes.c_uint32),
@@ -45,11 +54,13 @@ def oracle():
             getattr(o, f"or_permute_bst_{w}").argtypes = [VP, C, I]
             getattr(o, f"or_permute_btree_{w}").argtypes = [VP, C, C, I]
             getattr(o, f"or_permute_mveb_{w}").argtypes = [VP, C]
+            getattr(o, f"or_permute_classic_{w}").argtypes = [VP, C, I]
         for k in ("u32", "u64", "i32", "i64"):
             getattr(o, f"or_search_bst_{k}").argtypes = [VP, C, VP, C, VP, I]
             getattr(o, f"or_search_btree_{k}").argtypes = [VP, C, C, VP, C, VP, I]
             getattr(o, f"or_search_mveb_{k}").argtypes = [VP, C, VP, C, VP, I]
             getattr(o, f"or_search_binary_{k}").argtypes = [VP, C, VP, C, VP, I]
+            getattr(o, f"or_search_classic_{k}").argtypes = [VP, C, VP, C, VP, I]
         _o = o
     return _o
 
@@ -92,16 +103,19 @@ def permute(layout, a, b=16, p=1):
         rc = getattr(o, f"or_permute_bst_{w}")(_p(v), v.size, p)
     elif layout == "btree":
         rc = getattr(o, f"or_permute_btree_{w}")(_p(v), v.size, b, p)
+    elif layout == "veb":   # the classic (CPU) vEB
+        rc = getattr(o, f"or_permute_classic_{w}")(_p(v), v.size, p)
     else:
         rc = getattr(o, f"or_permute_mveb_{w}")(_p(v), v.size)
     assert rc == 0
     return a
 
 
-def layout_ranks(layout, n, b=16):
-    """Rank held by each slot (uint64), from the oracle's restated algorithm."""
+def layout_ranks(layout, n, b=16, p=1):
+    """Rank held by each slot (uint64), from the oracle's restated algorithm
+    (layout "bst", "btree", "mveb" or "veb", the classic CPU vEB)."""
     a = np.arange(n, dtype=np.uint64)
-    return permute(layout, a, b)
+    return permute(layout, a, b, p)
 
 
 def inorder_ranks(n, b):
@@ -121,6 +135,15 @@ def mveb_struct_ranks(n):
     return out
 
 
+def classic_struct_ranks(n):
+    """Independent construction of the classic vEB: the structural recursion"""
+    pos = np.zeros(n, dtype=np.uint64)
+    oracle().or_classic_layout(n, _p(pos))
+    out = np.empty(n, dtype=np.uint64)
+    out[pos] = np.arange(n, dtype=np.uint64)
+    return out
+
+
 def search(layout, A, Q, kind, b=16, p=8):
     o = oracle()
     A = np.ascontiguousarray(A, dtype=KIND_NP[kind])
@@ -130,6 +153,8 @@ def search(layout, A, Q, kind, b=16, p=8):
         getattr(o, f"or_search_bst_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     elif layout == "btree":
         getattr(o, f"or_search_btree_{kind}")(_p(A), A.size, b, _p(Q), Q.size, _p(ans), p)
+    elif layout == "veb":
+        getattr(o, f"or_search_classic_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     elif layout == "binary":   # A is the sorted array itself
         getattr(o, f"or_search_binary_{kind}")(_p(A), A.size, _p(Q), Q.size, _p(ans), p)
     else:
@@ -137,15 +162,16 @@ def search(layout, A, Q, kind, b=16, p=8):
     return ans
 
 
-def tableset(n):
-    """(d, num_tables, table_size, tables[list of (L,R,D)], idx, table_idx)"""
+def tableset(n, classic=False):
+    """(d, num_tables, table_size, tables[list of (L,R,D)], idx, table_idx) of
+    the mvEB, or of the classic vEB"""
     o = oracle()
     d = ctypes.c_uint32(); nt = ctypes.c_uint32(); ts = ctypes.c_uint32()
     tables = (VebTable * 4096)()
     idx = (ctypes.c_uint64 * 64)()
     tidx = (ctypes.c_uint32 * 64)()
-    assert o.or_veb_tableset(n, ctypes.byref(d), ctypes.byref(nt), ctypes.byref(ts), tables,
-                             idx, tidx) == 0
+    fn = o.or_classic_tableset if classic else o.or_veb_tableset
+    assert fn(n, ctypes.byref(d), ctypes.byref(nt), ctypes.byref(ts), tables, idx, tidx) == 0
     return (d.value, nt.value, ts.value, [(t.L, t.R, t.D) for t in tables[:ts.value]],
             list(idx[:nt.value]), list(tidx[:nt.value]))
 

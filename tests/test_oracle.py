@@ -166,6 +166,54 @@ def test_queries_brute_force_all_kinds():
                 assert np.array_equal(ans, exp), (kind, layout, b, n)
 
 
+def test_classic_veb_matches_reference_fixtures():
+    """The classic (CPU) vEB restatement against the compiled reference's
+    digests (every n <= 4096, 2^k-2 .. 2^k+1 up to 2^24, a few others)."""
+    with open(os.path.join(GOLD, "veb_classic_digests.json")) as f:
+        dig = json.load(f)
+    assert len(dig) > 4000
+    for key, want in dig.items():
+        n = int(key.split("_n")[1])
+        if n > 1 << 20:
+            continue   # the largest sizes run in the slow test below
+        assert ol.digest(ol.layout_ranks("veb", n, 0, p=8 if n > 4096 else 1)) == want, n
+
+
+@pytest.mark.slow
+def test_classic_veb_large_fixtures():
+    with open(os.path.join(GOLD, "veb_classic_digests.json")) as f:
+        dig = json.load(f)
+    for key, want in dig.items():
+        n = int(key.split("_n")[1])
+        if n > 1 << 20:
+            assert ol.digest(ol.layout_ranks("veb", n, 0, p=8)) == want, n
+
+
+def test_classic_veb_equals_structural_recursion():
+    for n in list(range(1, 600)) + [4095, 4096, 5000, 65535, 65536, 100003, 1 << 20]:
+        assert np.array_equal(ol.layout_ranks("veb", n, 0), ol.classic_struct_ranks(n)), n
+
+
+def test_classic_tables_match_reference_fixtures():
+    with open(os.path.join(GOLD, "veb_tables_classic.json")) as f:
+        want = json.load(f)
+    for d, exp in want.items():
+        d = int(d)
+        t = (ol.VebTable * d)()
+        ol.oracle().or_build_table_classic(t, (1 << d) - 1, d, 0)
+        assert [[int(e.L), int(e.R), int(e.D)] for e in t] == exp, d
+
+
+def test_classic_table_search_finds_every_key():
+    """Every key of the classic layout is found at its own slot with the
+    classic table set; keys in the gaps answer n."""
+    for n in list(range(1, 300)) + [1000, 4095, 4096, 5000, 65535, 65536, 100003]:
+        keys = np.arange(1, n + 1, dtype=np.uint64) * 3
+        A = keys[ol.layout_ranks("veb", n, 0).astype(np.int64)]
+        assert np.array_equal(ol.search("veb", A, A, "u64"), np.arange(n, dtype=np.uint64)), n
+        assert (ol.search("veb", A, keys + 1, "u64") == n).all(), n
+
+
 def test_binary_search_matches_reference_fixtures():
     data = np.load(os.path.join(GOLD, "binary_search.npz"))
     tags = sorted({k.rsplit("_", 1)[0] for k in data.files})
@@ -221,6 +269,14 @@ def test_live_reference_binary_search_agrees():
         ol.ref().ref_search_binary_u64(keys.ctypes.data_as(ol.VP), n, q.ctypes.data_as(ol.VP),
                                        q.size, ans.ctypes.data_as(ol.VP), 8)
         assert np.array_equal(ol.search("binary", keys, q, "u64"), ans), n
+
+
+@pytest.mark.skipif(ol.ref() is None, reason="reference not compiled in this environment")
+def test_live_reference_classic_veb_agrees():
+    for n in (1, 2, 3, 7, 100, 127, 128, 1000, 32767, 32768, 99999, 300007):
+        a = np.arange(n, dtype=np.uint64)
+        ol.ref().ref_permute_veb_u64(a.ctypes.data_as(ol.VP), n, 8)
+        assert np.array_equal(a, ol.layout_ranks("veb", n, 0)), n
 
 
 @pytest.mark.skipif(ol.ref() is None, reason="reference not compiled in this environment")
