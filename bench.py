@@ -265,6 +265,8 @@ def run_layout(args, dev, layout):
     ranks = rng.integers(0, n, size=100_000, dtype=np.int64)
     if layout == "bst":
         pos = np.array([o.or_btree_pos(int(x), n, 1) for x in ranks], dtype=np.int64)
+    elif layout == "veb":
+        pos = np.array([o.or_classic_pos(int(x), n) for x in ranks], dtype=np.int64)
     else:
         pos = np.array([o.or_mveb_pos(int(x), n) for x in ranks], dtype=np.int64)
     got = A[torch.from_numpy(pos).to(dev)].cpu().numpy()
@@ -290,13 +292,13 @@ def run_layout(args, dev, layout):
         def q():
             tl_amd.query_bst(A, Q, ans, kind="u64", timed=False)
     else:
-        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n)
-        num_tables = int(meta[0].item())
-        d = int(np.floor(np.log2(n))) + 1
+        classic = layout == "veb"
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, classic=classic)
+        d, num_tables = int(meta[0].item()), int(meta[1].item())
+        search = tl_amd.query_veb_nonperfect if classic else tl_amd.query_mveb_nonperfect
 
         def q():
-            tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind="u64",
-                                         timed=False)
+            search(A, tables, idx, tidx, d, num_tables, Q, ans, kind="u64", timed=False)
     q()
     torch.cuda.synchronize()
     ev0.record(stream)
@@ -338,7 +340,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--log2leaves", type=int, default=18, help="learned model leaves (log2)")
-    ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb"], default="btree")
+    ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb"],
+                    default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--batch-lds", type=int, default=0,
                     help="LDS budget of the batch's plans in bytes (0: the library default)")
@@ -353,7 +356,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "batch":
         return run_batch(args, dev)
-    if args.workload in ("bst", "mveb"):
+    if args.workload in ("bst", "mveb", "veb"):
         return run_layout(args, dev, args.workload)
     n, nq, b = 1 << args.log2n, 1 << args.log2q, args.b
     rng = np.random.default_rng(4)

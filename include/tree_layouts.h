@@ -59,6 +59,11 @@ tl_status tl_permute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype, 
  * timePermutevEB<T>(T *dev_A, uint64_t n)  (gpu/src/mveb-cycles.cuh:420). */
 tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
 
+/* Classic van Emde Boas layout, the reference's CPU-only layout (top tree of
+ * floor(d/2) levels, no power-of-two rounding).  Replaces
+ * timePermutevEB<T>(T *A, uint64_t n, uint32_t p)  (cpu/src/veb-cycles.h:381). */
+tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+
 /* ---- inverse permutations: layout -> sorted order, in place ------------- */
 
 /* The exact inverses of tl_permute_{bst,btree,mveb} for the same (n, b): the
@@ -70,6 +75,7 @@ tl_status tl_unpermute_bst(void *dev_A, uint64_t n, tl_dtype dtype, void *stream
 tl_status tl_unpermute_btree(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype, void *stream,
                              float *ms);
 tl_status tl_unpermute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
+tl_status tl_unpermute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
 
 /* ---- batched queries --------------------------------------------------- */
 
@@ -105,6 +111,21 @@ tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_ta
 tl_status tl_query_binary(const void *dev_A, uint64_t n, tl_dtype dtype, const void *dev_q,
                           uint64_t nq, uint64_t *dev_ans, void *stream, float *ms);
 
+/* Classic vEB queries (cpu/src/query-veb.h:31-52, 103-137): the same search
+ * with tables of the classic rule (tl_veb_classic_build_table,
+ * tl_veb_classic_tableset_device).  Replace searchAll / timeQueryvEB and
+ * searchAll_nonperfect / timeQueryvEB_nonperfect of cpu/src/query-veb.h:57,71,
+ * 141,155; the table set is packed as for the mvEB instead of the
+ * reference's array of per-structure pointers (cpu/src/query-veb.h:128). */
+tl_status tl_query_veb(const void *dev_A, const tl_veb_table *dev_table, uint64_t n, uint32_t d,
+                       tl_dtype dtype, const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                       void *stream, float *ms);
+tl_status tl_query_veb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                  const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                  uint64_t n, uint32_t d, uint32_t num_tables, tl_dtype dtype,
+                                  const void *dev_q, uint64_t nq, uint64_t *dev_ans,
+                                  void *stream, float *ms);
+
 /* ---- mvEB query tables -------------------------------------------------- */
 
 /* Host table build.  Replaces buildTable(vEB_table*, n, d, root_depth)
@@ -118,6 +139,14 @@ tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d, uint32
  *   dev_meta[0..2]: d, num_tables, table_size.                               */
 tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
                                  uint32_t *dev_table_idx, uint32_t *dev_meta, void *stream);
+
+/* The same two for the classic vEB: buildTable of cpu/src/query-veb.cpp:25
+ * and the table-set walk of cpu/test/query-veb-fixedN.cpp:37-158. */
+tl_status tl_veb_classic_build_table(tl_veb_table *table, uint64_t n, uint32_t d,
+                                     uint32_t root_depth);
+tl_status tl_veb_classic_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
+                                         uint32_t *dev_table_idx, uint32_t *dev_meta,
+                                         void *stream);
 
 /* ---- learned start node (no counterpart in the reference) -------------- */
 
@@ -190,8 +219,8 @@ tl_status tl_batch_destroy(void *handle);
 
 /* ---- plan inspection (no device needed) -------------------------------- */
 
-/* Dry run of tl_permute_{bst,btree,mveb} (layout 0, 1, 2; layout | 0x100 for
- * tl_unpermute_*): builds and bounds-checks the launch plan for (n, b, dtype)
+/* Dry run of tl_permute_{bst,btree,mveb,veb} (layout 0, 1, 2, 3; layout | 0x100
+ * for tl_unpermute_*): builds and bounds-checks the launch plan for (n, b, dtype)
  * with an LDS budget of lds_bytes
  * (0 = 64 KiB) and serializes it into buf (cap words; *used receives the
  * words needed).  Word layout: [ops, blob_words], then 32 words per launch,

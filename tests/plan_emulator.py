@@ -11,7 +11,7 @@ import oracle_lib as ol
 
 (OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
  OP_BITREV_LEVELS) = range(8)
-LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV = range(4)
+LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV, LDS_VEB = range(5)
 MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE, MV_ROW_SPREAD, MV_SPARSE_SPREAD = \
     range(6)
 
@@ -44,8 +44,9 @@ def lds_perm(kind, n, b, rows):
             ranks = ol.inorder_ranks(n, b).astype(np.int64)
             pos = np.empty(n, dtype=np.int64)
             pos[ranks] = x
-        elif kind == LDS_MVEB:
-            ranks = ol.mveb_struct_ranks(n).astype(np.int64)
+        elif kind in (LDS_MVEB, LDS_VEB):
+            ranks = (ol.mveb_struct_ranks(n) if kind == LDS_MVEB
+                     else ol.classic_struct_ranks(n)).astype(np.int64)
             pos = np.empty(n, dtype=np.int64)
             pos[ranks] = x
         elif kind == LDS_GATHER:

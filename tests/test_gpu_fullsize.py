@@ -121,12 +121,40 @@ def test_config3_mveb(n, dev):
     Q = torch.cat([Q, miss])
     is_hit = torch.isin(Q, keys).cpu().numpy()
     tables, idx, tidx, meta = tl_amd.veb_tableset_device(n)
-    num_tables = int(meta[0].item())
-    d = int(np.floor(np.log2(n))) + 1
+    d, num_tables = int(meta[0].item()), int(meta[1].item())
     ans = torch.empty(Q.numel(), dtype=torch.int64, device=dev)
     tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind="u64")
     _check_answers(A, Q, ans, n, is_hit)
     tl_amd.unpermute("mveb", A, kind="u64")
+    assert torch.equal(A, keys)
+
+
+@pytest.mark.parametrize("n", [(1 << 29) - 1, 3 * 10 ** 8])
+def test_classic_veb_fullsize(n, dev):
+    """The classic (CPU) vEB at the mvEB configuration's sizes and keys:
+    layout sample against the oracle's closed form, every hit found with the
+    classic table set, misses answer n, and the inverse restores the keys."""
+    keys, g = _gap_keys(n, 3, "geometric", dev)
+    A = keys.clone()
+    tl_amd.permute("veb", A, kind="u64")
+    assert _sorted_back_equal(A, keys)
+    o = ol.oracle()
+    rng = np.random.default_rng(5)
+    ranks = rng.integers(0, n, size=200_000, dtype=np.int64)
+    pos = np.array([o.or_classic_pos(int(x), n) for x in ranks], dtype=np.int64)
+    assert torch.equal(A[torch.from_numpy(pos).to(dev)], keys[torch.from_numpy(ranks).to(dev)])
+    Q = torch.cat([keys[torch.randint(0, n, (1 << 24,), generator=g, device=dev)],
+                   keys[torch.randint(0, n - 1, (1 << 20,), generator=g, device=dev)] + 1])
+    is_hit = torch.isin(Q, keys).cpu().numpy()
+    tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, classic=True)
+    d, nt = int(meta[0].item()), int(meta[1].item())
+    ans = torch.empty(Q.numel(), dtype=torch.int64, device=dev)
+    if nt == 1 and n == (1 << d) - 1:
+        tl_amd.query_veb(A, tables, d, Q, ans, kind="u64")
+    else:
+        tl_amd.query_veb_nonperfect(A, tables, idx, tidx, d, nt, Q, ans, kind="u64")
+    _check_answers(A, Q, ans, n, is_hit)
+    tl_amd.unpermute("veb", A, kind="u64")
     assert torch.equal(A, keys)
 
 

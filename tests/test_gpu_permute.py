@@ -32,7 +32,7 @@ def _run(layout, b, kind, n, dev):
 
 
 CASES = [("bst", 1), ("btree", 16), ("btree", 8), ("btree", 2), ("btree", 3), ("btree", 32),
-         ("btree", 1), ("mveb", 0)]
+         ("btree", 1), ("mveb", 0), ("veb", 0)]
 
 
 @pytest.mark.parametrize("layout,b", CASES)
@@ -65,7 +65,8 @@ def test_permute_stash_budgets(layout, b, row_stash, dev, monkeypatch):
 @pytest.mark.parametrize("layout,b,n", [("bst", 1, (1 << 24) - 1), ("bst", 1, (1 << 24) + 12345),
                                         ("btree", 16, 17 ** 6 - 1), ("btree", 16, 30_000_001),
                                         ("btree", 8, 9 ** 7 + 5), ("mveb", 0, (1 << 24) - 1),
-                                        ("mveb", 0, 30_000_001)])
+                                        ("mveb", 0, 30_000_001), ("veb", 0, (1 << 23) - 1),
+                                        ("veb", 0, 30_000_001)])
 def test_permute_large(layout, b, n, dev):
     got, exp = _run(layout, b, "u64", n, dev)
     assert np.array_equal(got, exp), (layout, b, n)

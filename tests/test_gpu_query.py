@@ -54,10 +54,13 @@ def gpu_answers(layout, b, kind, A_np, Q_np, dev):
     elif layout == "btree":
         tl_amd.query_btree(A, b, Q, ans, kind=kind)
     else:
-        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev)
+        classic = layout == "veb"
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev, classic=classic)
         d, nt, ts = (int(v) for v in meta[:3].cpu())
         if nt == 1 and n == (1 << d) - 1:
-            tl_amd.query_mveb(A, tables, d, Q, ans, kind=kind)
+            (tl_amd.query_veb if classic else tl_amd.query_mveb)(A, tables, d, Q, ans, kind=kind)
+        elif classic:
+            tl_amd.query_veb_nonperfect(A, tables, idx, tidx, d, nt, Q, ans, kind=kind)
         else:
             tl_amd.query_mveb_nonperfect(A, tables, idx, tidx, d, nt, Q, ans, kind=kind)
     torch.cuda.synchronize()
@@ -65,7 +68,7 @@ def gpu_answers(layout, b, kind, A_np, Q_np, dev):
 
 
 QCASES = [("bst", 1), ("btree", 16), ("btree", 8), ("btree", 2), ("btree", 5), ("btree", 32),
-          ("btree", 64), ("btree", 1), ("btree", 100), ("mveb", 0)]
+          ("btree", 64), ("btree", 1), ("btree", 100), ("mveb", 0), ("veb", 0)]
 
 
 @pytest.mark.parametrize("layout,b", QCASES)
@@ -108,12 +111,13 @@ def test_binary_search_matches_oracle(kind, dev):
         assert np.array_equal(got, exp), (kind, n)
 
 
-def test_tableset_device_matches_oracle(dev):
+@pytest.mark.parametrize("classic", [False, True])
+def test_tableset_device_matches_oracle(classic, dev):
     for n in [1, 2, 3, 6, 7, 100, 127, 1000, 65535, 65536, 100000, 300_000_000, (1 << 29) - 1,
               3 * 10 ** 8, (1 << 30)]:
-        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev)
+        tables, idx, tidx, meta = tl_amd.veb_tableset_device(n, device=dev, classic=classic)
         d, nt, ts = (int(v) for v in meta[:3].cpu())
-        od, ont, ots, otab, oidx, otidx = ol.tableset(n)
+        od, ont, ots, otab, oidx, otidx = ol.tableset(n, classic=classic)
         assert (d, nt, ts) == (od, ont, ots), n
         raw = tables.cpu().numpy().view(np.uint64).reshape(-1, 3)[:ts]
         got = [(int(r[0]), int(r[1]), int(r[2]) & 0xFFFFFFFF) for r in raw]

@@ -20,7 +20,7 @@ SMALL = [0, 1, 2, 3, 4, 7, 8, 15, 16, 17, 31, 63, 100, 255, 256, 1000, 1023, 409
 MEDIUM = [65535, 65536, 83520, 83521, 100003, 262143, 300007, 1048575, 1048576, 1419856,
           1419857, 2000003]
 CASES = [("bst", 1), ("btree", 16), ("btree", 8), ("btree", 2), ("btree", 3), ("btree", 32),
-         ("btree", 1), ("mveb", 0)]
+         ("btree", 1), ("mveb", 0), ("veb", 0)]
 
 
 def _check(layout, b, kind, n, dev, rng):
@@ -67,7 +67,8 @@ def test_unpermute_stash_budgets(layout, b, row_stash, dev, monkeypatch):
 @pytest.mark.parametrize("layout,b,n", [("bst", 1, (1 << 24) - 1), ("bst", 1, (1 << 24) + 12345),
                                         ("btree", 16, 17 ** 6 - 1), ("btree", 16, 30_000_001),
                                         ("btree", 8, 9 ** 7 + 5), ("mveb", 0, (1 << 24) - 1),
-                                        ("mveb", 0, 30_000_001)])
+                                        ("mveb", 0, 30_000_001), ("veb", 0, (1 << 23) - 1),
+                                        ("veb", 0, 30_000_001)])
 def test_unpermute_large(layout, b, n, dev):
     A = torch.arange(n, dtype=torch.int64, device=dev)
     tl_amd.permute(layout, A, b=b, kind="u64")

@@ -32,7 +32,7 @@ def test_invalid_arguments_fail_without_device():
 
 
 @pytest.mark.parametrize("layout,bs", [("btree", (16, 8, 2, 3, 32, 1)), ("bst", (1,)),
-                                       ("mveb", (0,))])
+                                       ("mveb", (0,)), ("veb", (0,))])
 @pytest.mark.parametrize("lds", [256, 2048])
 @pytest.mark.parametrize("row_stash", [None, "0", "40"])
 def test_plans_replay_to_oracle(layout, bs, lds, row_stash, monkeypatch):
@@ -55,14 +55,15 @@ def test_plans_replay_to_oracle(layout, bs, lds, row_stash, monkeypatch):
 def test_full_size_plans_are_in_bounds():
     for layout, b, n in [("btree", 16, 1 << 30), ("btree", 16, 10 ** 8), ("bst", 1, 1 << 30),
                          ("bst", 1, 3 << 28), ("mveb", 0, (1 << 29) - 1),
-                         ("mveb", 0, 3 * 10 ** 8), ("mveb", 0, 1 << 30), ("btree", 8, 1 << 22)]:
+                         ("mveb", 0, 3 * 10 ** 8), ("mveb", 0, 1 << 30), ("btree", 8, 1 << 22),
+                         ("veb", 0, (1 << 29) - 1), ("veb", 0, 3 * 10 ** 8), ("veb", 0, 1 << 30)]:
         for kind in ("u32", "u64"):
             buf = tl_amd.plan_export(layout, n, b, kind)
             assert int(buf[0]) > 0
 
 
 @pytest.mark.parametrize("layout,bs", [("btree", (16, 8, 2, 3, 32, 1)), ("bst", (1,)),
-                                       ("mveb", (0,))])
+                                       ("mveb", (0,)), ("veb", (0,))])
 @pytest.mark.parametrize("lds", [256, 2048])
 @pytest.mark.parametrize("row_stash", [None, "0", "40"])
 def test_inverse_plans_replay_to_identity(layout, bs, lds, row_stash, monkeypatch):
@@ -112,7 +113,8 @@ def test_chained_moves_wait_for_every_overwritten_source(inverse):
     import batch_emulator as bem
     cases = [("btree", 16, 10 ** 6), ("btree", 16, 17 ** 4 + 12345), ("btree", 2, 300001),
              ("bst", 1, 1 << 21), ("bst", 1, 3 << 19), ("mveb", 0, (1 << 24) - 1),
-             ("mveb", 0, (1 << 24) + 12345), ("mveb", 0, 5 * 10 ** 6)]
+             ("mveb", 0, (1 << 24) + 12345), ("mveb", 0, 5 * 10 ** 6),
+             ("veb", 0, (1 << 23) - 1), ("veb", 0, (1 << 24) + 12345)]
     kinds_seen = set()
     for layout, b, n in cases:
         buf = tl_amd.plan_export(layout, n, b, "u64", 0, inverse=inverse)
@@ -158,3 +160,20 @@ def test_chained_moves_wait_for_every_overwritten_source(inverse):
         assert {pe.MV_ROW_SPREAD, pe.MV_SPARSE_SPREAD, pe.MV_SHIFT_LEFT} <= kinds_seen
     else:
         assert {pe.MV_ROW_SLIDE, pe.MV_SPARSE_SLIDE, pe.MV_SHIFT_RIGHT} <= kinds_seen
+
+
+def test_host_table_builds_match_oracle():
+    """tl_veb_build_table (mvEB rule) and tl_veb_classic_build_table (classic
+    rule, checked against the compiled reference's tables) on the host."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "veb_tables_classic.json")) as f:
+        want = json.load(f)
+    for d, exp in want.items():
+        d = int(d)
+        got = tl_amd.veb_build_table((1 << d) - 1, d, classic=True)
+        assert [[int(a), int(b), int(c)] for a, b, c in got] == exp, d
+    for d in range(1, 40):
+        t = (ol.VebTable * d)()
+        ol.oracle().or_build_table(t, (1 << d) - 1, d, 0)
+        got = tl_amd.veb_build_table((1 << d) - 1, d)
+        assert got == [(e.L, e.R, e.D) for e in t], d

@@ -92,6 +92,8 @@ __device__ void ex_lds(const BOp &op, uint64_t sub, const uint64_t *blob, unsign
             p = btree_pos32(x, sh);
         else if (kind == LDS_MVEB)
             p = mveb_pos32(x, n);
+        else if (kind == LDS_VEB)
+            p = mveb_pos32(x, n, true);
         else if (kind == LDS_GATHER)
             p = gather_pos32(x, rows, b);
         else {

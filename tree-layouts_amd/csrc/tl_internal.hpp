@@ -105,6 +105,19 @@ __host__ __device__ inline uint32_t floor_log2_64(uint64_t n) {
 // Split of a depth-d tree into a top tree of depth root_d and bottom trees of
 // depth leaf_d, leaf_d rounded up to a power of two
 // (gpu/src/mveb-cycles.cuh:240-247).
+// classic split of the CPU vEB (cpu/src/veb-cycles.h:114-115,
+// cpu/src/query-veb.cpp:28-29): top tree floor(d/2) levels, bottom trees the
+// rest; `classic` = false is the modified vEB's split below
+__host__ __device__ inline void mveb_split(uint32_t d, uint32_t &rd, uint32_t &ld);
+__host__ __device__ inline void veb_split(uint32_t d, uint32_t &rd, uint32_t &ld, bool classic) {
+    if (classic) {
+        rd = (d - 2) / 2 + 1;
+        ld = d - rd;
+    } else {
+        mveb_split(d, rd, ld);
+    }
+}
+
 __host__ __device__ inline void mveb_split(uint32_t d, uint32_t &rd, uint32_t &ld) {
     uint32_t r = (d - 2) / 2 + 1;
     uint32_t l = d - r;
@@ -183,12 +196,12 @@ __host__ __device__ inline uint32_t btree_pos32(uint32_t x, const BtreeShape &s)
 
 // slot of rank x in the modified vEB of n keys (structural recursion of
 // gpu/src/mveb-cycles.cuh:230-414)
-__host__ __device__ inline uint32_t mveb_pos32(uint32_t x, uint32_t n) {
+__host__ __device__ inline uint32_t mveb_pos32(uint32_t x, uint32_t n, bool classic = false) {
     uint32_t base = 0;
     for (;;) {
         if (n <= 1) return base;
         uint32_t d = floor_log2_64(n) + 1, rd, ld;
-        mveb_split(d, rd, ld);
+        veb_split(d, rd, ld, classic);
         uint32_t r = (1u << rd) - 1, l = (1u << ld) - 1;
         uint64_t full = (1ull << d) - 1;
         uint32_t nf, inc, span;
@@ -233,6 +246,7 @@ __host__ __device__ inline uint32_t gather_pos32(uint32_t x, uint32_t rows, uint
     return (c == b) ? row : rows + row * b + c;
 }
 
-enum LdsKind : uint32_t { LDS_BTREE = 0, LDS_MVEB = 1, LDS_GATHER = 2 };
+// in-LDS layout kinds (3 is the BST top levels' per-level bit reversal)
+enum LdsKind : uint32_t { LDS_BTREE = 0, LDS_MVEB = 1, LDS_GATHER = 2, LDS_VEB = 4 };
 
 }   // namespace tl

@@ -46,6 +46,7 @@ __device__ inline uint32_t lds_slot(uint32_t x, uint32_t n, uint32_t kind, uint3
                                     uint32_t rows, const BtreeShape &sh) {
     if (kind == LDS_BTREE) return btree_pos32(x, sh);
     if (kind == LDS_MVEB) return mveb_pos32(x, n);
+    if (kind == LDS_VEB) return mveb_pos32(x, n, true);
     if (kind == LDS_GATHER) return gather_pos32(x, rows, b);
     const uint32_t y = x + 1, t = 31u - __clz(y);
     return ((1u << t) | rev_low_bits32(y & ((1u << t) - 1), t)) - 1;
@@ -582,6 +583,16 @@ extern "C" tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, vo
     return tl::permute(tl::L_MVEB, dev_A, n, 0, dtype, stream, ms);
 }
 
+extern "C" tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                    float *ms) {
+    return tl::permute(tl::L_VEB, dev_A, n, 0, dtype, stream, ms);
+}
+
+extern "C" tl_status tl_unpermute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                      float *ms) {
+    return tl::permute(tl::L_VEB, dev_A, n, 0, dtype, stream, ms, true);
+}
+
 extern "C" tl_status tl_unpermute_bst(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
                                       float *ms) {
     return tl::permute(tl::L_BST, dev_A, n, 1, dtype, stream, ms, true);
@@ -602,8 +613,9 @@ extern "C" tl_status tl_plan_export(uint32_t layout, uint64_t n, uint64_t b, tl_
                                     uint64_t *used) {
     const bool inverse = (layout & tl::PLAN_INVERSE) != 0;
     layout &= ~tl::PLAN_INVERSE;
-    if (layout > 2) {
-        tl::set_error("layout must be 0 (BST), 1 (B-tree) or 2 (mvEB), optionally | 0x100");
+    if (layout > 3) {
+        tl::set_error("layout must be 0 (BST), 1 (B-tree), 2 (mvEB) or 3 (classic vEB), "
+                      "optionally | 0x100");
         return TL_EINVAL;
     }
     return tl::export_plan((tl::Layout)layout, n, b, dtype, lds_bytes, buf, cap, used, inverse);

@@ -483,7 +483,8 @@ struct VebBatch {
 };
 
 inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
-                          std::vector<VebBatch> &leaves_lds, VebBatch B, uint32_t depth) {
+                          std::vector<VebBatch> &leaves_lds, VebBatch B, uint32_t depth,
+                          bool classic) {
     if (B.n <= 1 || B.runs.empty()) return;
     if (B.n <= P.cap()) {
         leaves_lds.push_back(B);
@@ -497,7 +498,7 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
     const uint64_t n = B.n;
     const uint32_t d = floor_log2_64(n) + 1;
     uint32_t rd, ld;
-    mveb_split(d, rd, ld);
+    veb_split(d, rd, ld, classic);
     const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
     std::vector<VebBatch> kids;
     if (n == (1ull << d) - 1) {
@@ -542,21 +543,23 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
     P.pl.max_stash = std::max(P.pl.max_stash, sub.max_stash);
     P.pl.max_moves = std::max(P.pl.max_moves, sub.max_moves);
     for (auto &op : sub.ops) depth_ops[depth].push_back(op);
-    for (auto &k : kids) plan_mveb_rec(P, depth_ops, leaves_lds, k, depth + 1);
+    for (auto &k : kids) plan_mveb_rec(P, depth_ops, leaves_lds, k, depth + 1, classic);
 }
 
-inline void plan_mveb(Planner &P, uint64_t n) {
+// classic = true: the CPU vEB layout (cpu/src/veb-cycles.h:34-160), the same
+// recursion with the classic split
+inline void plan_mveb(Planner &P, uint64_t n, bool classic = false) {
     if (n <= 1) return;
     std::vector<std::vector<Op>> depth_ops;
     std::vector<VebBatch> leaves;
-    plan_mveb_rec(P, depth_ops, leaves, VebBatch{n, {{0, 0, 1}}}, 0);
+    plan_mveb_rec(P, depth_ops, leaves, VebBatch{n, {{0, 0, 1}}}, 0, classic);
     for (auto &ops : depth_ops)
         for (auto &op : ops) P.pl.ops.push_back(op);
     // group LDS leaves by size
     std::map<uint64_t, std::vector<Run>> by_n;
     for (auto &lb : leaves)
         for (auto &r : lb.runs) by_n[lb.n].push_back(r);
-    for (auto &kv : by_n) P.lds(kv.second, kv.first, LDS_MVEB);
+    for (auto &kv : by_n) P.lds(kv.second, kv.first, classic ? LDS_VEB : LDS_MVEB);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,7 +660,7 @@ inline bool width_of(tl_dtype dt, uint32_t &W) {
     return false;
 }
 
-enum Layout { L_BST, L_BTREE, L_MVEB };
+enum Layout { L_BST, L_BTREE, L_MVEB, L_VEB };
 
 // flag in a layout number: the inverse permutation (layout -> sorted order)
 constexpr uint32_t PLAN_INVERSE = 0x100;
@@ -717,7 +720,7 @@ inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint
     else if (lay == L_BTREE)
         plan_btree(P, n, b);
     else
-        plan_mveb(P, n);
+        plan_mveb(P, n, lay == L_VEB);
 }
 
 }   // namespace tl

@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
 // device table-set build: one lane walks the chain of incomplete bottom trees
 // (gpu/test/query-mveb-fixedN.cu:42-165) and runs buildTable
 // (gpu/src/query-mveb.cu:25-47) for every structure with an explicit stack.
-__device__ void build_table_dev(tl_veb_table *table, uint64_t n, uint32_t d) {
+__device__ void build_table_dev(tl_veb_table *table, uint64_t n, uint32_t d, bool classic) {
     struct Fr {
         uint32_t base, d, root_depth;
         uint64_t n;
@@ -317,7 +317,7 @@ __device__ void build_table_dev(tl_veb_table *table, uint64_t n, uint32_t d) {
         const Fr f = st[top--];
         if (f.n <= 1) continue;
         uint32_t rd, ld;
-        mveb_split(f.d, rd, ld);
+        veb_split(f.d, rd, ld, classic);
         const uint64_t root_n = (1ull << rd) - 1, leaf_n = (1ull << ld) - 1;
         const uint32_t i = f.d - ld;
         table[f.base + i].L = leaf_n;
@@ -328,8 +328,10 @@ __device__ void build_table_dev(tl_veb_table *table, uint64_t n, uint32_t d) {
     }
 }
 
+// classic = true builds the classic vEB's table set with the classic split
+// (cpu/src/query-veb.cpp:25-41, cpu/test/query-veb-fixedN.cpp:37-158)
 __global__ void k_tableset(uint64_t n, tl_veb_table *tables, uint64_t *idx, uint32_t *tidx,
-                           uint32_t *meta) {
+                           uint32_t *meta, bool classic) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const uint32_t d = floor_log2_64(n) + 1;
     uint64_t sizes[64];
@@ -342,7 +344,7 @@ __global__ void k_tableset(uint64_t n, tl_veb_table *tables, uint64_t *idx, uint
         uint32_t td = d;
         while (td >= 2 && nt < 64) {
             uint32_t rd, ld;
-            mveb_split(td, rd, ld);
+            veb_split(td, rd, ld, classic);
             const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
             const uint64_t nf = (tn - r) / l, inc = tn - r - nf * l;
             if (inc == 0) break;
@@ -360,7 +362,7 @@ __global__ void k_tableset(uint64_t n, tl_veb_table *tables, uint64_t *idx, uint
     uint32_t off = 0;
     for (uint32_t j = 0; j < nt; j++) {
         tidx[j] = off;
-        build_table_dev(tables + off, sizes[j], depths[j]);
+        build_table_dev(tables + off, sizes[j], depths[j], classic);
         off += depths[j];
         idx[j] = (j + 1 < nt) ? n - sizes[j + 1] : n;
     }
@@ -372,7 +374,7 @@ __global__ void k_tableset(uint64_t n, tl_veb_table *tables, uint64_t *idx, uint
 // ---------------------------------------------------------------------------
 // host side
 
-static uint32_t tableset_size(uint64_t n) {
+static uint32_t tableset_size(uint64_t n, bool classic) {
     const uint32_t d = floor_log2_64(n) + 1;
     uint32_t tsize = d;
     if (n == (1ull << d) - 1) return tsize;
@@ -380,7 +382,7 @@ static uint32_t tableset_size(uint64_t n) {
     uint32_t td = d, nt = 1;
     while (td >= 2 && nt < 64) {
         uint32_t rd, ld;
-        mveb_split(td, rd, ld);
+        veb_split(td, rd, ld, classic);
         const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
         const uint64_t nf = (tn - r) / l, inc = tn - r - nf * l;
         if (inc == 0) break;
@@ -547,22 +549,21 @@ extern "C" tl_status tl_query_mveb(const void *dev_A, const tl_veb_table *dev_ta
     return tm.stop();
 }
 
-extern "C" tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
-                                              const uint64_t *dev_idx, const uint32_t *dev_table_idx,
-                                              uint64_t n, uint32_t d, uint32_t num_tables,
-                                              tl_dtype dtype, const void *dev_q, uint64_t nq,
-                                              uint64_t *dev_ans, void *stream, float *ms) {
+static tl_status query_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                  const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                  uint64_t n, uint32_t num_tables, tl_dtype dtype,
+                                  const void *dev_q, uint64_t nq, uint64_t *dev_ans, void *stream,
+                                  float *ms, bool classic) {
     TL_TRY(qcheck(dev_A, n, dev_q, nq, dev_ans));
     if (nq == 0) return TL_OK;
     if (n == 0 || num_tables == 0 || num_tables > 64 || !dev_tables || !dev_idx ||
         !dev_table_idx) {
-        set_error("tl_query_mveb_nonperfect: bad table set");
+        set_error("non-perfect vEB query: bad table set");
         return TL_EINVAL;
     }
     // table_size is not part of the reference signature; it is fixed by n
     // (the sum of the structure depths of the chain of incomplete trees)
-    const uint32_t table_size = tableset_size(n);
-    (void)d;
+    const uint32_t table_size = tableset_size(n, classic);
     Ctx *c;
     TL_TRY(get_ctx(&c));
     hipStream_t s = (hipStream_t)stream;
@@ -574,13 +575,41 @@ extern "C" tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_ta
     return tm.stop();
 }
 
-extern "C" tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d,
-                                        uint32_t root_depth) {
+extern "C" tl_status tl_query_mveb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                              const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                              uint64_t n, uint32_t d, uint32_t num_tables,
+                                              tl_dtype dtype, const void *dev_q, uint64_t nq,
+                                              uint64_t *dev_ans, void *stream, float *ms) {
+    (void)d;
+    return query_nonperfect(dev_A, dev_tables, dev_idx, dev_table_idx, n, num_tables, dtype,
+                            dev_q, nq, dev_ans, stream, ms, false);
+}
+
+extern "C" tl_status tl_query_veb(const void *dev_A, const tl_veb_table *dev_table, uint64_t n,
+                                  uint32_t d, tl_dtype dtype, const void *dev_q, uint64_t nq,
+                                  uint64_t *dev_ans, void *stream, float *ms) {
+    // the search of a perfect tree depends only on its table (classic rule)
+    return tl_query_mveb(dev_A, dev_table, n, d, dtype, dev_q, nq, dev_ans, stream, ms);
+}
+
+extern "C" tl_status tl_query_veb_nonperfect(const void *dev_A, const tl_veb_table *dev_tables,
+                                             const uint64_t *dev_idx, const uint32_t *dev_table_idx,
+                                             uint64_t n, uint32_t d, uint32_t num_tables,
+                                             tl_dtype dtype, const void *dev_q, uint64_t nq,
+                                             uint64_t *dev_ans, void *stream, float *ms) {
+    (void)d;
+    return query_nonperfect(dev_A, dev_tables, dev_idx, dev_table_idx, n, num_tables, dtype,
+                            dev_q, nq, dev_ans, stream, ms, true);
+}
+
+static tl_status build_table_host(tl_veb_table *table, uint64_t n, uint32_t d,
+                                  uint32_t root_depth, bool classic) {
     if (!table || d == 0 || d > 63) {
-        set_error("tl_veb_build_table: bad arguments");
+        set_error("vEB table build: bad arguments");
         return TL_EINVAL;
     }
-    // host recursion of gpu/src/query-mveb.cu:25-47
+    // host recursion of gpu/src/query-mveb.cu:25-47 (classic:
+    // cpu/src/query-veb.cpp:25-41)
     struct Fr {
         tl_veb_table *t;
         uint64_t n;
@@ -592,7 +621,7 @@ extern "C" tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_
         st.pop_back();
         if (f.n <= 1) continue;
         uint32_t rd, ld;
-        mveb_split(f.d, rd, ld);
+        veb_split(f.d, rd, ld, classic);
         const uint64_t root_n = (1ull << rd) - 1, leaf_n = (1ull << ld) - 1;
         const uint32_t i = f.d - ld;
         f.t[i].L = leaf_n;
@@ -604,16 +633,38 @@ extern "C" tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_
     return TL_OK;
 }
 
-extern "C" tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
-                                            uint32_t *dev_table_idx, uint32_t *dev_meta,
-                                            void *stream) {
+extern "C" tl_status tl_veb_build_table(tl_veb_table *table, uint64_t n, uint32_t d,
+                                        uint32_t root_depth) {
+    return build_table_host(table, n, d, root_depth, false);
+}
+
+extern "C" tl_status tl_veb_classic_build_table(tl_veb_table *table, uint64_t n, uint32_t d,
+                                                uint32_t root_depth) {
+    return build_table_host(table, n, d, root_depth, true);
+}
+
+static tl_status tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
+                                 uint32_t *dev_table_idx, uint32_t *dev_meta, void *stream,
+                                 bool classic) {
     if (n == 0 || !dev_tables || !dev_idx || !dev_table_idx || !dev_meta) {
-        set_error("tl_veb_tableset_device: bad arguments");
+        set_error("vEB table set: bad arguments");
         return TL_EINVAL;
     }
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_tableset, dim3(1), dim3(64), 0, s, n, dev_tables, dev_idx, dev_table_idx,
-                       dev_meta);
+                       dev_meta, classic);
     TL_LAUNCH_CHECK();
     return TL_OK;
+}
+
+extern "C" tl_status tl_veb_classic_tableset_device(uint64_t n, tl_veb_table *dev_tables,
+                                                    uint64_t *dev_idx, uint32_t *dev_table_idx,
+                                                    uint32_t *dev_meta, void *stream) {
+    return tableset_device(n, dev_tables, dev_idx, dev_table_idx, dev_meta, stream, true);
+}
+
+extern "C" tl_status tl_veb_tableset_device(uint64_t n, tl_veb_table *dev_tables, uint64_t *dev_idx,
+                                            uint32_t *dev_table_idx, uint32_t *dev_meta,
+                                            void *stream) {
+    return tableset_device(n, dev_tables, dev_idx, dev_table_idx, dev_meta, stream, false);
 }

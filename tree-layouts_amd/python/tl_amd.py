@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtl_amd.so")
 TL_OK, TL_EINVAL, TL_ENOMEM, TL_EHIP, TL_ETIMEOUT, TL_EUNSUPPORTED = range(6)
 DTYPE = {"u32": 0, "u64": 1, "i32": 2, "i64": 3}
 WIDTH = {"u32": 4, "u64": 8, "i32": 4, "i64": 8}
-LAYOUT = {"bst": 0, "btree": 1, "mveb": 2}
+LAYOUT = {"bst": 0, "btree": 1, "mveb": 2, "veb": 3}   # "veb": the classic (CPU) vEB
 
 
 class TLError(RuntimeError):
@@ -49,6 +49,10 @@ _SIGS = {
                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
     "tl_unpermute_mveb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                           ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_veb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                       ctypes.POINTER(ctypes.c_float)],
+    "tl_unpermute_veb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                         ctypes.POINTER(ctypes.c_float)],
     "tl_query_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                      ctypes.POINTER(ctypes.c_float)],
@@ -66,6 +70,18 @@ _SIGS = {
                                  ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
                                  ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_float)],
+    "tl_query_veb": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                     ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_query_veb_nonperfect": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.POINTER(ctypes.c_float)],
+    "tl_veb_classic_build_table": [ctypes.POINTER(VebTable), ctypes.c_uint64, ctypes.c_uint32,
+                                   ctypes.c_uint32],
+    "tl_veb_classic_tableset_device": [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "tl_veb_build_table": [ctypes.POINTER(VebTable), ctypes.c_uint64, ctypes.c_uint32,
                            ctypes.c_uint32],
     "tl_veb_tableset_device": [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
@@ -188,11 +204,20 @@ def permute_mveb(A, kind=None, stream=None, timed=True):
                        timed=timed)
 
 
+def permute_veb(A, kind=None, stream=None, timed=True):
+    """The classic (CPU) vEB layout."""
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_permute_veb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                       timed=timed)
+
+
 def permute(layout, A, b=16, kind=None, stream=None, timed=True):
     if layout == "bst":
         return permute_bst(A, kind, stream, timed)
     if layout == "btree":
         return permute_btree(A, b, kind, stream, timed)
+    if layout == "veb":
+        return permute_veb(A, kind, stream, timed)
     return permute_mveb(A, kind, stream, timed)
 
 
@@ -206,6 +231,9 @@ def unpermute(layout, A, b=16, kind=None, stream=None, timed=True):
     if layout == "btree":
         return _timed_call(L.tl_unpermute_btree, _ptr(A), A.numel(), b, DTYPE[k],
                            _stream(stream), timed=timed)
+    if layout == "veb":
+        return _timed_call(L.tl_unpermute_veb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
+                           timed=timed)
     return _timed_call(L.tl_unpermute_mveb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
                        timed=timed)
 
@@ -246,22 +274,40 @@ def query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None
                        _ptr(ans), _stream(stream), timed=timed)
 
 
-def veb_tableset_device(n, device="cuda", stream=None):
-    """Device-built mvEB table set: (tables int64[64*64*3], idx, tidx, meta) tensors."""
+def veb_tableset_device(n, device="cuda", stream=None, classic=False):
+    """Device-built table set of the mvEB (or, classic=True, of the classic
+    vEB): (tables int64[64*64*3], idx, tidx, meta) tensors."""
     import torch
     tables = torch.zeros(64 * 64 * 3, dtype=torch.int64, device=device)   # 24-byte entries
     idx = torch.zeros(64, dtype=torch.int64, device=device)
     tidx = torch.zeros(64, dtype=torch.int32, device=device)
     meta = torch.zeros(4, dtype=torch.int32, device=device)
-    _check(lib().tl_veb_tableset_device(n, _ptr(tables), _ptr(idx), _ptr(tidx), _ptr(meta),
-                                        _stream(stream)))
+    fn = lib().tl_veb_classic_tableset_device if classic else lib().tl_veb_tableset_device
+    _check(fn(n, _ptr(tables), _ptr(idx), _ptr(tidx), _ptr(meta), _stream(stream)))
     return tables, idx, tidx, meta
 
 
-def veb_build_table(n, d, root_depth=0):
+def veb_build_table(n, d, root_depth=0, classic=False):
     arr = (VebTable * max(d, 1))()
-    _check(lib().tl_veb_build_table(arr, n, d, root_depth))
+    fn = lib().tl_veb_classic_build_table if classic else lib().tl_veb_build_table
+    _check(fn(arr, n, d, root_depth))
     return [(e.L, e.R, e.D) for e in arr]
+
+
+def query_veb(A, table, d, Q, ans, kind=None, stream=None, timed=True):
+    """Classic vEB of a perfect tree, with a classic-rule table."""
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_veb, _ptr(A), _ptr(table), A.numel(), d, DTYPE[k],
+                       _ptr(Q), Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+
+
+def query_veb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None, stream=None,
+                         timed=True):
+    """Classic vEB of any size, with the classic table set."""
+    k = _kind(A, kind)
+    return _timed_call(lib().tl_query_veb_nonperfect, _ptr(A), _ptr(tables), _ptr(idx),
+                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _ptr(Q), Q.numel(),
+                       _ptr(ans), _stream(stream), timed=timed)
 
 
 def plan_export(layout, n, b=1, kind="u64", lds_bytes=0, inverse=False):
