@@ -10,7 +10,7 @@ import numpy as np
 import oracle_lib as ol
 
 (OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
- OP_BITREV_LEVELS) = range(8)
+ OP_BITREV_LEVELS, OP_ROTATE) = range(9)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV, LDS_VEB = range(5)
 MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE, MV_ROW_SPREAD, MV_SPARSE_SPREAD = \
     range(6)
@@ -112,6 +112,9 @@ def replay(buf, A, stash=None):
                 par = np.int64((1 << lev) - 1) + rev_bits(y, lev)
                 old = A.copy()
                 A[idx] = old[par]
+        elif t == OP_ROTATE:
+            P, L, k = p[0], p[1], p[2]
+            A[P:P + L] = np.roll(A[P:P + L].copy(), k)
         elif t == OP_UNIT_MERGE:
             unit, cs_at, ncyc, cu_at, inv = p[0], p[1], p[2], p[3], p[4]
             cs = blob[cs_at: cs_at + ncyc + 1]

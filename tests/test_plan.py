@@ -177,3 +177,57 @@ def test_host_table_builds_match_oracle():
         ol.oracle().or_build_table(t, (1 << d) - 1, d, 0)
         got = tl_amd.veb_build_table((1 << d) - 1, d)
         assert got == [(e.L, e.R, e.D) for e in t], d
+
+
+def _walk_model(L, c, w, K=8):
+    """The lane walk of the one-pass rotation (k_rotate_walk), lane by lane:
+    returns the rotated array and how many times each position was written,
+    and checks that no lane reads a position another lane already wrote."""
+    A = list(range(L))
+    orig = list(A)
+    stash = orig[:w]
+    writes = [0] * L
+    owner = [None] * L
+    for j in range(w):
+        dst = j
+        while True:
+            s, stop, d = [], False, dst
+            for _ in range(K):
+                src = d - c if d >= c else d + L - c
+                s.append(src)
+                if src < w:
+                    stop = True
+                    break
+                d = src
+            vals = []
+            for k, src in enumerate(s):
+                if stop and k == len(s) - 1:
+                    vals.append(stash[src])
+                else:
+                    assert owner[src] is None, ("read after another lane wrote", L, c, w, j, src)
+                    vals.append(A[src])
+            for k, v in enumerate(vals):
+                pos = dst if k == 0 else s[k - 1]
+                A[pos] = v
+                writes[pos] += 1
+                owner[pos] = j
+            if stop:
+                break
+            dst = s[-1]
+    return A, writes
+
+
+def test_rotation_lane_walk_model():
+    """Every (L, c) up to L = 40 and every lane count w >= gcd(L, c): the walk
+    writes each position once and yields the rotation (numpy.roll)."""
+    import math
+    for L in range(2, 41):
+        for c in range(1, L):
+            g = math.gcd(L, c)
+            for w in sorted({g, g + 1, min(L, 2 * g + 3), L - 1, L}):
+                if w < g or w < 1 or w > L:
+                    continue
+                for K in (1, 3, 8):
+                    A, writes = _walk_model(L, c, w, K)
+                    assert writes == [1] * L, (L, c, w, K)
+                    assert A == list(np.roll(np.arange(L), c)), (L, c, w, K)

@@ -626,8 +626,10 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     B.blob.clear();
     B.nflags = 0;
     uint64_t stash_elems = 0;
-    Tuning tu;
-    tu.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
+    Tuning tu_persistent;
+    tu_persistent.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
+    // the persistent executors rotate by reversals (they have no lane walk)
+    tu_persistent.rotate_walk = false;
     struct Pending {
         uint32_t op, depth, array;
     };
@@ -643,7 +645,7 @@ static tl_status build_batch(Batch &B, Ctx *c) {
         }
         Plan pl;
         const uint64_t b = it.layout == 0 ? 1 : it.b;
-        build_plan(pl, tu, (Layout)it.layout, it.n, b, W);
+        build_plan(pl, tu_persistent, (Layout)it.layout, it.n, b, W);
         std::string why;
         if (!validate_plan(pl, it.n, why)) {
             set_error(why);

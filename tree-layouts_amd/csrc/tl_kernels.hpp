@@ -447,6 +447,52 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
 }
 
 // ---------------------------------------------------------------------------
+// one-pass rotation of A[0, L) right by c (key at p goes to p + c mod L) by
+// parallel cycle leaders.  The first w keys are stashed beforehand; lane j
+// (one thread) writes j, j - c, j - 2c, ... (mod L), each from the position c
+// before it, until that position is a lane start (< w), whose key comes from
+// the stash.  Every position is written by exactly one lane (the first start
+// reached walking forward from it, which exists when w >= gcd(L, c)) after that
+// lane read it, so lanes never wait for each other.  Neighbouring lanes touch
+// neighbouring keys, so a wave's accesses are contiguous; each lane keeps K
+// loads in flight.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t L, uint64_t c,
+                                                     uint64_t w, const T *__restrict__ stash) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= w) return;
+    const uint64_t back = L - c;   // p - c (mod L) = p + back when p < c
+    uint64_t dst = j;
+    for (;;) {
+        uint64_t s[K];
+        T v[K];
+        int m = 0;
+        bool stop = false;
+        uint64_t d = dst;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!stop) {
+                const uint64_t src = d >= c ? d - c : d + back;
+                s[k] = src;
+                m = k + 1;
+                if (src < w)
+                    stop = true;
+                else
+                    d = src;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (k < m) v[k] = (stop && k == m - 1) ? stash[s[k]] : A[s[k]];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (k < m) A[k == 0 ? dst : s[k - 1]] = v[k];
+        if (stop) return;
+        dst = s[K - 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // strided copy (stash / unstash of displaced keys)
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_strided(T *__restrict__ dst, uint64_t dstride,

@@ -333,6 +333,17 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             TL_LAUNCH_CHECK();
             break;
         }
+        case OP_ROTATE: {
+            const uint64_t P = op.p[0], L = op.p[1], k = op.p[2], w = op.p[3];
+            const uint32_t g = (uint32_t)std::min<uint64_t>((w + 255) / 256, 65535);
+            hipLaunchKernelGGL(k_copy_strided<T>, dim3(g), dim3(256), 0, s, stash, 1,
+                               (const T *)(A + P), 1, w);
+            TL_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_rotate_walk<T, 4>), dim3((uint32_t)((w + 255) / 256)), dim3(256),
+                               0, s, A + P, L, k, w, (const T *)stash);
+            TL_LAUNCH_CHECK();
+            break;
+        }
         case OP_UNIT_MERGE: {
             const uint64_t unit = op.p[0];
             const uint32_t nchunks = (uint32_t)((unit + 255) / 256);

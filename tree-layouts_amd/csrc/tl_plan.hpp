@@ -48,7 +48,8 @@ enum OpType {
     OP_SPARSE_STREAM,
     OP_COPY,
     OP_CHAINED,
-    OP_BITREV_LEVELS   // the per-level reversals of levels p[0] .. p[1]-1, tiles 2^p[2]
+    OP_BITREV_LEVELS,  // the per-level reversals of levels p[0] .. p[1]-1, tiles 2^p[2]
+    OP_ROTATE          // [p0, p0+p1) rotated right by p2 in one pass, p3 lanes (stash)
 };
 
 struct Op {
@@ -119,6 +120,9 @@ struct Tuning {
     int64_t row_stash_env = -1;            // $TL_ROW_STASH_MAX when set (0: no row slides)
     uint64_t pack_bytes = 32768;           // keys of several small in-LDS items per workgroup
     uint32_t bitrev_k = 0;                 // bit-reversal tile 2^K x 2^K (0: 6 for 4-byte keys, 5 for 8-byte)
+    bool rotate_walk = true;               // one-pass rotations (else three reversals)
+    uint64_t walk_lanes = 1ull << 24;      // lanes (stashed keys) of a one-pass rotation
+    int64_t walk_env = -1;                 // $TL_WALK_LANES when set (0: no one-pass rotations)
 };
 
 inline void tuning_env(Tuning &t) {
@@ -126,6 +130,17 @@ inline void tuning_env(Tuning &t) {
     if (r) t.row_stash_env = atoll(r);
     const char *k = getenv("TL_BITREV_K");
     if (k && (atoi(k) == 5 || atoi(k) == 6)) t.bitrev_k = (uint32_t)atoi(k);
+    const char *w = getenv("TL_WALK_LANES");
+    if (w) t.walk_env = atoll(w);
+}
+
+inline uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
 }
 
 inline Tuning tuning_for(Ctx *c) {
@@ -231,6 +246,25 @@ struct Planner {
                 copy(P + k, 1, x, false);
             }
             return;
+        }
+        // one pass: w lanes start at the first w keys (stashed), each walks
+        // its part of a cycle of p -> p + k (mod len); every cycle needs a
+        // start, so w >= gcd(len, k).  About 16 steps per lane keep the stash
+        // traffic at 1/8 of a pass.
+        if (items == 1 && tu.rotate_walk) {
+            const uint64_t g = gcd64(len, k);
+            const uint64_t w = std::min<uint64_t>(tu.walk_lanes, std::max<uint64_t>(len / 16, g));
+            if (w >= g && w >= 1) {
+                Op op;
+                op.type = OP_ROTATE;
+                op.p[0] = runs[0].start;
+                op.p[1] = len;
+                op.p[2] = k;
+                op.p[3] = w;
+                pl.max_stash = std::max(pl.max_stash, w);
+                pl.ops.push_back(op);
+                return;
+            }
         }
         // three reversals: whole, then the two parts
         Op a;
@@ -597,6 +631,12 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             end = (1ull << d) - 1;
             break;
         }
+        case OP_ROTATE:
+            end = op.p[0] + op.p[1];
+            if (op.p[2] == 0 || op.p[2] >= op.p[1] || op.p[3] == 0 || op.p[3] > op.p[1] ||
+                op.p[3] > pl.max_stash || op.p[3] < gcd64(op.p[1], op.p[2]))
+                neg = true;
+            break;
         case OP_UNIT_MERGE: {
             uint64_t mx = 0;
             const uint64_t ncyc = op.p[2];
@@ -669,7 +709,8 @@ constexpr uint32_t PLAN_INVERSE = 0x100;
 // reverse order, each replaced by its inverse.  Reversals and bit-reversal
 // tiles are involutions; LDS layouts, unit merges and streamed root gathers
 // run with their inverse flag; copies swap direction; a shift becomes the
-// opposite shift of the moved region; a row slide becomes a row spread (which
+// opposite shift of the moved region; a one-pass rotation by k becomes the
+// rotation by len - k (it fills its own stash); a row slide becomes a row spread (which
 // takes the separators back from the stash the inverted copy before it
 // fills); a sparse slide becomes a sparse spread (its roots are put back by
 // the inverted copies around it).
@@ -689,6 +730,7 @@ inline void invert_plan(const Plan &f, Plan &g) {
         case OP_REVERSE:
         case OP_BITREV:
         case OP_BITREV_LEVELS: break;
+        case OP_ROTATE: op.p[2] = op.p[1] - op.p[2]; break;
         case OP_CHAINED: {
             MoveSpec &m = op.ms;
             if (m.kind == MV_SHIFT_RIGHT) {
@@ -714,6 +756,13 @@ inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint
     Tuning tu = tu0;
     tu.row_stash_max = tu.row_stash_env >= 0 ? (uint64_t)tu.row_stash_env
                                               : std::max<uint64_t>(tu.stash_max, n / 16);
+    // one-pass rotations use the same budget: more lanes, shorter walks
+    if (tu.walk_env >= 0) {
+        tu.rotate_walk = tu.rotate_walk && tu.walk_env > 0;
+        tu.walk_lanes = (uint64_t)tu.walk_env;
+    } else {
+        tu.walk_lanes = std::max<uint64_t>(tu.stash_max, n / 16);
+    }
     Planner P{pl, tu, W};
     if (lay == L_BST)
         plan_bst(P, n);
