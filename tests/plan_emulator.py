@@ -127,9 +127,15 @@ def replay(buf, A, stash=None):
                         dst = us[(k - 1) % len(us)] if inv else us[(k + 1) % len(us)]
                         A[o + dst * unit: o + (dst + 1) * unit] = vals[k]
         elif t == OP_SPARSE_STREAM:
-            r, l, inv = p[0], p[1], p[2]
+            r, l, inv, fuse = p[0], p[1], p[2], p[3]
+            kind = LDS_MVEB if fuse == 1 else LDS_VEB
             for o in items(op, blob):
                 if inv:
+                    if fuse:   # undo the kids' layouts first (r + 1 leaf blocks)
+                        sub = A[o:o + r].copy()
+                        A[o:o + r] = sub[lds_perm(kind, r, 0, 0)]
+                        blk = A[o + r: o + r + (r + 1) * l].reshape(r + 1, l)
+                        blk[:, :] = blk[:, lds_perm(kind, l, 0, 0)].copy()
                     roots = A[o:o + r].copy()
                     leaves = A[o + r: o + r * (l + 1)].reshape(r, l).copy()
                     reg = A[o:o + r * (l + 1)].reshape(r, l + 1)
@@ -139,6 +145,13 @@ def replay(buf, A, stash=None):
                 reg = A[o:o + r * (l + 1)].reshape(r, l + 1).copy()
                 A[o:o + r] = reg[:, l]
                 A[o + r: o + r * (l + 1)] = reg[:, :l].reshape(-1)
+                if fuse:   # the roots and all r + 1 leaf blocks laid out in place
+                    sub = A[o:o + r].copy()
+                    A[o + lds_perm(kind, r, 0, 0)] = sub
+                    blk = A[o + r: o + r + (r + 1) * l].reshape(r + 1, l)
+                    new = np.empty_like(blk)
+                    new[:, lds_perm(kind, l, 0, 0)] = blk
+                    blk[:, :] = new
         elif t == OP_COPY:
             P, stride, count, to_stash, soff = p[0], p[1], p[2], p[3], p[4]
             if to_stash:

@@ -630,6 +630,7 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     tu_persistent.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
     // the persistent executors rotate by reversals (they have no lane walk)
     tu_persistent.rotate_walk = false;
+    tu_persistent.fuse_leaves = false;
     struct Pending {
         uint32_t op, depth, array;
     };

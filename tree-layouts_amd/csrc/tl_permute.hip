@@ -270,6 +270,123 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
     for (uint64_t i = threadIdx.x; i < r; i += blockDim.x) a[i] = roots[i];
 }
 
+// The same gather with the layouts of its kids fused in (fuse = 1 mvEB,
+// 2 classic vEB) over a whole perfect subtree, r rows of (l, 1) followed by
+// the last leaf block: the r roots become a vEB of r keys at the front and
+// every leaf block (the last one in place) a vEB of l keys, so the in-LDS
+// pass over the kids disappears.
+// Batches hold whole rows, so every row's destination block (inside which
+// its keys are scattered) lies right of the batch start; the inverse walks
+// the same way from the left with the slot map inverted in LDS.  Needs
+// l + 1 <= 256 * U.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sparse_stream_fused(T *__restrict__ A,
+                                                             const uint64_t *__restrict__ runs,
+                                                             uint32_t nruns, uint32_t r, uint32_t l,
+                                                             uint32_t inv, uint32_t fuse) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *roots = reinterpret_cast<T *>(smem_raw);
+    uint32_t *slot = reinterpret_cast<uint32_t *>(smem_raw + (((size_t)r * sizeof(T) + 15) & ~15ull));
+    T *a = A + run_offset(runs, nruns, blockIdx.x);
+    const bool classic = fuse == 2;
+    const uint32_t L1 = l + 1;
+    constexpr int U = 8;
+    const uint32_t rows_per = (256u * U) / L1;
+    // slot[x]: where rank x of a leaf block goes (its inverse when inv)
+    for (uint32_t x = threadIdx.x; x < l; x += blockDim.x) {
+        const uint32_t q = mveb_pos32(x, l, classic);
+        if (inv)
+            slot[q] = x;
+        else
+            slot[x] = q;
+    }
+    if (inv)
+        for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) roots[i] = a[mveb_pos32(i, r, classic)];
+    __syncthreads();
+    if (inv) {
+        {
+            T v[U];
+            const uint64_t b = (uint64_t)r * L1;
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                if (y < l) v[k] = a[b + y];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                if (y < l) a[b + slot[y]] = v[k];
+            }
+        }
+        for (uint32_t lo = 0; lo < r; lo += rows_per) {
+            const uint32_t hi = min(r, lo + rows_per);
+            const uint32_t n_el = (hi - lo) * l;
+            T v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                if (y < n_el) v[k] = a[r + (uint64_t)lo * l + y];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                if (y < n_el) {
+                    const uint32_t rr = y / l, q = y - rr * l, row = lo + rr;
+                    a[(uint64_t)row * L1 + slot[q]] = v[k];
+                    if (q == 0) a[(uint64_t)row * L1 + l] = roots[row];
+                }
+            }
+        }
+        return;
+    }
+    {
+        // the last leaf block, which the gather leaves in place: a[b + slot[y]]
+        // takes the key at b + y (forward: rank y to its slot; inverse: the
+        // key at slot y back to its rank)
+        T v[U];
+        const uint64_t b = (uint64_t)r * L1;
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+            if (y < l) v[k] = a[b + y];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+            if (y < l) a[b + slot[y]] = v[k];
+        }
+    }
+    uint32_t hi = r;
+    while (hi > 0) {
+        const uint32_t lo = hi > rows_per ? hi - rows_per : 0;
+        const uint32_t n_el = (hi - lo) * L1;
+        T v[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint32_t x = threadIdx.x + (uint32_t)k * 256u;
+            if (x < n_el) v[k] = a[(uint64_t)lo * L1 + x];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const uint32_t x = threadIdx.x + (uint32_t)k * 256u;
+            if (x < n_el) {
+                const uint32_t rr = x / L1, cc = x - rr * L1, row = lo + rr;
+                if (cc == l)
+                    roots[row] = v[k];
+                else
+                    a[r + (uint64_t)row * l + slot[cc]] = v[k];
+            }
+        }
+        hi = lo;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) a[mveb_pos32(i, r, classic)] = roots[i];
+}
+
 // ---------------------------------------------------------------------------
 // executor
 
@@ -357,9 +474,15 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
         }
         case OP_SPARSE_STREAM: {
             const uint64_t r = op.p[0], l = op.p[1];
-            hipLaunchKernelGGL(k_sparse_stream_runs<T>, dim3((uint32_t)op.nitems), dim3(256),
-                               r * sizeof(T), s, A, runs, (uint32_t)op.nruns, r, l,
-                               (uint32_t)op.p[2]);
+            if (op.p[3])
+                hipLaunchKernelGGL(k_sparse_stream_fused<T>, dim3((uint32_t)op.nitems), dim3(256),
+                                   ((r * sizeof(T) + 15) & ~15ull) + l * 4, s, A, runs,
+                                   (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
+                                   (uint32_t)op.p[2], (uint32_t)op.p[3]);
+            else
+                hipLaunchKernelGGL(k_sparse_stream_runs<T>, dim3((uint32_t)op.nitems), dim3(256),
+                                   r * sizeof(T), s, A, runs, (uint32_t)op.nruns, r, l,
+                                   (uint32_t)op.p[2]);
             TL_LAUNCH_CHECK();
             break;
         }

@@ -121,6 +121,7 @@ struct Tuning {
     uint64_t pack_bytes = 32768;           // keys of several small in-LDS items per workgroup
     uint32_t bitrev_k = 0;                 // bit-reversal tile 2^K x 2^K (0: 6 for 4-byte keys, 5 for 8-byte)
     bool rotate_walk = true;               // one-pass rotations (else three reversals)
+    bool fuse_leaves = true;               // vEB leaf layouts fused into streamed gathers
     uint64_t walk_lanes = 1ull << 24;      // lanes (stashed keys) of a one-pass rotation
     int64_t walk_env = -1;                 // $TL_WALK_LANES when set (0: no one-pass rotations)
 };
@@ -397,8 +398,11 @@ struct Planner {
 
     // ---- sparse gather for the mvEB: every item holds r rows of (l leaf keys,
     //      1 root) followed by untouched keys
-    void gather_sparse(const std::vector<Run> &runs, uint64_t r, uint64_t l) {
-        if (runs.empty() || r == 0) return;
+    // returns true when the kids' layouts were fused into the gather (fuse:
+    // 1 mvEB, 2 classic vEB; only for a streamed gather of short rows whose
+    // kids both fit in LDS)
+    bool gather_sparse(const std::vector<Run> &runs, uint64_t r, uint64_t l, uint32_t fuse = 0) {
+        if (runs.empty() || r == 0) return false;
         uint64_t items = 0;
         for (auto &x : runs) items += x.count;
         const uint64_t part = r * (l + 1);
@@ -411,11 +415,14 @@ struct Planner {
             pl.put_runs(op, runs);
             op.p[0] = r;
             op.p[1] = l;
+            const bool fused = fuse && tu.fuse_leaves && l + 1 <= 2048 && r <= cap() && l <= cap();
+            op.p[3] = fused ? fuse : 0;
             pl.ops.push_back(op);
-            return;
+            return fused;
         }
         for (const Run &x : runs)
             for (uint64_t i = 0; i < x.count; i++) sparse_slide(x.start + i * x.stride, r, l, 0, 0);
+        return false;
     }
 
     // one chained pass over r rows of (l leaf keys, 1 root) at P, followed by
@@ -536,7 +543,14 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
     const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
     std::vector<VebBatch> kids;
     if (n == (1ull << d) - 1) {
-        Q.gather_sparse(B.runs, r, l);
+        if (Q.gather_sparse(B.runs, r, l, classic ? 2u : 1u)) {
+            // both kids were laid out inside the gather
+            sub.blob.swap(P.pl.blob);
+            P.pl.max_stash = std::max(P.pl.max_stash, sub.max_stash);
+            P.pl.max_moves = std::max(P.pl.max_moves, sub.max_moves);
+            for (auto &op : sub.ops) depth_ops[depth].push_back(op);
+            return;
+        }
         bool contiguous = (rd == ld);
         for (auto &x : B.runs)
             if (x.count > 1 && x.stride != n) contiguous = false;
@@ -645,7 +659,10 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             end = runs_end(pl, op, (mx + 1) * op.p[0]);
             break;
         }
-        case OP_SPARSE_STREAM: end = runs_end(pl, op, op.p[0] * (op.p[1] + 1)); break;
+        case OP_SPARSE_STREAM:   // a fused gather also covers the last leaf block
+            end = runs_end(pl, op, op.p[0] * (op.p[1] + 1) + (op.p[3] ? op.p[1] : 0));
+            if (op.p[3] > 2 || (op.p[3] && (op.p[1] + 1 > 2048 || op.p[0] * 8 > 65536))) neg = true;
+            break;
         case OP_COPY:
             end = op.p[0] + (op.p[2] ? (op.p[2] - 1) * op.p[1] + 1 : 0);
             if (op.p[4] + op.p[2] > pl.max_stash) neg = true;
