@@ -468,6 +468,8 @@ def main():
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_GBs": None if traffic is None
                      else round(traffic / (perm_ms * 1e-3) / 1e9, 1),
+                     "traffic_frac": None if traffic is None
+                     else round(traffic / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "kernel": "one tl_permute_btree call (all its launches); traffic from "
                                "profiles/btree_b16_n2p30_u64_permute_traffic.txt"},
         "cpu_baseline": None if cpu is None else {
