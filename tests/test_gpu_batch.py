@@ -91,12 +91,15 @@ def test_batch_separate_path_matches(dev):
     B.close()
 
 
+@pytest.mark.parametrize("waves", ["1", "0"])
 @pytest.mark.parametrize("lds", [0, 32768])
-def test_batch_planned_path_matches_oracle(lds, dev):
-    """tl_batch_run (the planned per-array path: small arrays in batched LDS
-    launches, the others on their own plans, queries per array) matches the
-    oracle eagerly and on three replays of its HIP graph, with the arrays
-    restored and the program's own work and synchronizations in between."""
+def test_batch_planned_path_matches_oracle(lds, waves, dev, monkeypatch):
+    """tl_batch_run (the planned path: small arrays in batched LDS launches,
+    the others by waves of multi-array launches, or each on its own plan with
+    TL_BATCH_WAVES=0; queries per array) matches the oracle eagerly and on
+    three replays of its HIP graph, with the arrays restored and the
+    program's own work and synchronizations in between."""
+    monkeypatch.setenv("TL_BATCH_WAVES", waves)
     arrays, host = make_batch(240, 0, 18, 5, dev)
     B = tl_amd.Batch(arrays, lds_bytes=lds)
     B.run(graph=False)

@@ -508,6 +508,13 @@ struct LargeItem {
     uint64_t at = 0;              // byte offset of the plan blob
     uint64_t flags_at = 0;        // byte offset of its chained-move flags
 };
+// one step of the wave-merged permutation: a multi-array launch, or (for op
+// kinds without a multi-array kernel) one array's single op
+struct WaveStep {
+    bool multi = true;
+    MultiLaunch m{};
+    uint32_t large = 0, op = 0;   // single op: large-item index and op index
+};
 struct QTables {                  // device mvEB table set of one array
     uint64_t tables_at = 0, idx_at = 0, tidx_at = 0;
     uint32_t nt = 0;
@@ -521,6 +528,13 @@ struct Batch {
     uint32_t plan_lds = 0;        // LDS budget of the plans; 0 = the device's whole budget
     void *d_pdata = nullptr, *d_pstash = nullptr;
     uint32_t *d_pflags = nullptr;
+    // wave-merged permutation: the k-th op of every array in a few launches
+    bool waves = true;
+    std::vector<WaveStep> steps;
+    std::vector<uint64_t> wstash_at;   // per large item: byte offset of its stash
+    void *d_wdata = nullptr, *d_wstash = nullptr;
+    uint32_t *d_wctr = nullptr;
+    uint64_t nctr = 0;
     uint64_t small_items = 0;
     // concurrent branches: each has its own stash and flags
     static constexpr int NSUB = 4;
@@ -550,7 +564,7 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : {(void *)d_ops, (void *)d_tasks, (void *)d_blob, (void *)d_done,
                         (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash, d_pdata,
-                        d_pstash, (void *)d_pflags})
+                        d_pstash, (void *)d_pflags, d_wdata, d_wstash, (void *)d_wctr})
             if (p) (void)hipFree(p);
         for (int k = 0; k < NSUB; k++)
             if (sub[k]) (void)hipStreamDestroy(sub[k]);
@@ -895,6 +909,177 @@ static tl_status build_planned(Batch &B, Ctx *c) {
     return TL_OK;
 }
 
+// Groups the k-th op of every large array by kernel (kind, key width and
+// template parameter) into one launch each.  Within a wave every op belongs
+// to a different array, so the launches of a wave are independent; a
+// rotation's stash copy runs with the wave's copies and its lane walk after
+// them.  Reversals and unit merges (rare in batches) run as single ops.
+static tl_status build_waves(Batch &B, Ctx *c) {
+    const unsigned char *pbase = (const unsigned char *)B.d_pdata;
+    // a stash of its own for every array: their ops run together
+    uint64_t wst = 0;
+    B.wstash_at.assign(B.large.size(), 0);
+    for (size_t i = 0; i < B.large.size(); i++) {
+        B.wstash_at[i] = wst;
+        wst += (B.large[i].plan.max_stash * B.large[i].W + 255) & ~255ull;
+    }
+    TL_HIP(hipMalloc(&B.d_wstash, std::max<uint64_t>(wst, 256)));
+    const uint64_t sbase = (uint64_t)B.d_wstash;
+    size_t maxops = 0;
+    for (const LargeItem &L : B.large) maxops = std::max(maxops, L.plan.ops.size());
+    struct Group {
+        MultiLaunch m{};
+        std::vector<unsigned char> desc;
+        std::vector<uint64_t> pre{0};
+    };
+    std::vector<unsigned char> wdata;
+    auto put = [&wdata](const void *src, size_t bytes) -> uint64_t {
+        const uint64_t at = (wdata.size() + 63) & ~63ull;
+        wdata.resize(at + bytes);
+        if (bytes) memcpy(wdata.data() + at, src, bytes);
+        return at;
+    };
+    struct Pending {
+        WaveStep st;
+        uint64_t desc_at = 0, pre_at = 0;
+    };
+    std::vector<Pending> pend;
+    uint32_t nchain = 0;
+    for (size_t k = 0; k < maxops; k++) {
+        std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> ga, gb;
+        std::vector<Pending> singles;
+        auto add = [](std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> &g,
+                      uint32_t kind, uint32_t W, uint32_t param, uint32_t threads, const void *d,
+                      size_t dsize, uint64_t tasks, size_t lds) {
+            Group &G = g[std::make_tuple(kind, W, param, threads)];
+            G.m.kind = kind;
+            G.m.W = W;
+            G.m.param = param;
+            G.m.threads = threads;
+            G.m.lds = std::max(G.m.lds, lds);
+            const size_t at = G.desc.size();
+            G.desc.resize(at + dsize);
+            memcpy(G.desc.data() + at, d, dsize);
+            G.pre.push_back(G.pre.back() + tasks);
+        };
+        for (size_t li = 0; li < B.large.size(); li++) {
+            const LargeItem &L = B.large[li];
+            if (k >= L.plan.ops.size()) continue;
+            const Op &op = L.plan.ops[k];
+            const uint32_t W = L.W;
+            const uint64_t A = (uint64_t)B.items[L.item].A;
+            const uint64_t runs = (uint64_t)(pbase + L.at + op.runs_at * 8);
+            const uint64_t stash = sbase + B.wstash_at[li];
+            switch (op.type) {
+            case OP_COPY: {
+                MCopy d{};
+                const uint64_t Ap = A + op.p[0] * W, St = stash + op.p[4] * W;
+                if (op.p[3])
+                    d = MCopy{St, Ap, 1, op.p[1], op.p[2]};
+                else
+                    d = MCopy{Ap, St, op.p[1], 1, op.p[2]};
+                add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[2] + 2047) / 2048, 0);
+                break;
+            }
+            case OP_ROTATE: {
+                const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3]};
+                add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
+                const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3]};
+                add(gb, MK_WALK, W, 0, 256, &w, sizeof w, (op.p[3] + 255) / 256, 0);
+                break;
+            }
+            case OP_LDS: {
+                const uint64_t n = op.p[0], per = op.p[4];
+                const MLds d{A, runs, op.nitems, (uint32_t)op.nruns, (uint32_t)per, (uint32_t)n,
+                             (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3],
+                             (uint32_t)((0xFFFFFFFFull + n) / n), (uint32_t)op.p[5]};
+                const uint32_t threads = (per * n >= 8192) ? 512 : 256;
+                add(ga, MK_LDS, W, 0, threads, &d, sizeof d, (op.nitems + per - 1) / per,
+                    per * n * W + (per > 1 ? n * 4 : 0));
+                break;
+            }
+            case OP_SPARSE_STREAM: {
+                const uint64_t r = op.p[0], l = op.p[1];
+                const MStream d{A, runs, (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
+                                (uint32_t)op.p[2], (uint32_t)op.p[3], 0};
+                const bool fused = op.p[3] != 0;
+                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, 256, &d, sizeof d, op.nitems,
+                    fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
+                break;
+            }
+            case OP_CHAINED: {
+                // this op's flag region, laid out as launch_plan lays them out
+                uint64_t off = 0;
+                for (size_t q = 0; q < k; q++)
+                    if (L.plan.ops[q].type == OP_CHAINED)
+                        off += ((L.plan.ops[q].p[0] + 15) / 16) * 16 + 16;
+                const MChain d{A, (uint64_t)B.d_pflags + L.flags_at + off * 4, stash, op.ms};
+                const uint32_t mode = op.ms.kind == MV_ROW_SPREAD      ? (uint32_t)CH_ROW_SPREAD
+                                      : op.ms.kind == MV_SPARSE_SPREAD ? (uint32_t)CH_SPARSE_SPREAD
+                                                                       : (uint32_t)CH_FORWARD;
+                add(ga, MK_CHAIN, W, mode, 256, &d, sizeof d, op.p[0], 0);
+                break;
+            }
+            case OP_BITREV: {
+                const MBitrev d{A, (int64_t)op.p[0], (uint32_t)op.p[1], 0, 0, 0};
+                add(ga, MK_BITREV, W, (uint32_t)op.p[2], 256, &d, sizeof d,
+                    1ull << (op.p[1] - 2 * op.p[2]), 0);
+                break;
+            }
+            case OP_BITREV_LEVELS: {
+                const MBitrev d{A, 0, 0, (uint32_t)op.p[0], (uint32_t)op.p[1], 1};
+                add(ga, MK_BITREV, W, (uint32_t)op.p[2], 256, &d, sizeof d,
+                    (1ull << (op.p[1] - op.p[0])) - 1, 0);
+                break;
+            }
+            default: {
+                Pending p;
+                p.st.multi = false;
+                p.st.large = (uint32_t)li;
+                p.st.op = (uint32_t)k;
+                singles.push_back(p);
+            }
+            }
+        }
+        auto flush = [&](std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> &g) {
+            for (auto &kv : g) {
+                Group &G = kv.second;
+                Pending p;
+                p.st.m = G.m;
+                p.st.m.nd = (uint32_t)(G.pre.size() - 1);
+                p.st.m.tasks = G.pre.back();
+                p.desc_at = put(G.desc.data(), G.desc.size());
+                p.pre_at = put(G.pre.data(), G.pre.size() * 8);
+                if (G.m.kind == MK_CHAIN) p.st.m.counter = nchain++;
+                pend.push_back(p);
+            }
+        };
+        flush(ga);
+        for (Pending &p : singles) pend.push_back(p);
+        flush(gb);
+    }
+    TL_HIP(hipMalloc(&B.d_wdata, std::max<size_t>(wdata.size(), 64)));
+    if (!wdata.empty())
+        TL_HIP(hipMemcpy(B.d_wdata, wdata.data(), wdata.size(), hipMemcpyHostToDevice));
+    B.nctr = std::max<uint64_t>(nchain, 1) * 16;   // 64 bytes per ticket
+    TL_HIP(hipMalloc((void **)&B.d_wctr, B.nctr * 4));
+    const uint64_t wbase = (uint64_t)B.d_wdata;
+    B.steps.clear();
+    for (Pending &p : pend) {
+        if (p.st.multi) {
+            p.st.m.desc = wbase + p.desc_at;
+            p.st.m.pre = wbase + p.pre_at;
+            if (p.st.m.kind == MK_CHAIN) p.st.m.counter = (uint64_t)B.d_wctr + p.st.m.counter * 64;
+            if (p.st.m.lds > c->lds_max - LDS_STATIC_RESERVE) {
+                set_error("tl_batch: a wave launch needs more LDS than the device has");
+                return TL_EINVAL;
+            }
+        }
+        B.steps.push_back(p.st);
+    }
+    return TL_OK;
+}
+
 // the branches start after everything enqueued on s and s continues after all
 // of them (events; under capture this makes the graph's parallel branches)
 static tl_status fork_branches(Batch &B, int f, hipStream_t s) {
@@ -910,10 +1095,34 @@ static tl_status join_branches(Batch &B, int f, hipStream_t s) {
     return TL_OK;
 }
 
+static tl_status run_queries(Batch &B, const unsigned char *base, hipStream_t s);
+
 static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
     const unsigned char *base = (const unsigned char *)B.d_pdata;
     TL_TRY(fill_u32_launch(c->err_flag, 1, 0u, s));
     TL_TRY(fill_u32_launch(B.d_pflags, B.flag_stride / 4, 0u, s));
+    if (B.waves) {
+        // permutations wave by wave, every launch over many arrays
+        TL_TRY(fill_u32_launch(B.d_wctr, B.nctr, 0u, s));
+        for (size_t g = 0; g < B.small.size(); g++) {
+            const SmallGroup &G = B.small[g];
+            TL_TRY(lds_items_launch((const uint64_t *)(base + G.at), (uint32_t)(G.desc.size() / 2),
+                                    G.W, G.kind, G.b, G.max_n, s));
+        }
+        for (const WaveStep &st : B.steps) {
+            if (st.multi) {
+                TL_TRY(multi_launch(c, st.m, s));
+                continue;
+            }
+            const LargeItem &L = B.large[st.large];
+            Plan one;
+            one.ops.push_back(L.plan.ops[st.op]);
+            one.max_stash = L.plan.max_stash;
+            TL_TRY(plan_launch(c, one, B.items[L.item].A, L.W, s, (const uint64_t *)(base + L.at),
+                               (unsigned char *)B.d_wstash + B.wstash_at[st.large], nullptr));
+        }
+        return run_queries(B, base, s);
+    }
     // permutations: every array on one branch, with that branch's stash and flags
     TL_TRY(fork_branches(B, 0, s));
     for (size_t g = 0; g < B.small.size(); g++) {
@@ -930,7 +1139,11 @@ static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
                            (uint32_t *)((unsigned char *)B.d_pflags + L.flags_at)));
     }
     TL_TRY(join_branches(B, 0, s));
-    // queries, once every layout is in place
+    return run_queries(B, base, s);
+}
+
+// every array's query batch, once every layout is in place
+static tl_status run_queries(Batch &B, const unsigned char *base, hipStream_t s) {
     TL_TRY(fork_branches(B, 1, s));
     for (uint32_t a = 0; a < B.items.size(); a++) {
         const tl_batch_item &it = B.items[a];
@@ -981,6 +1194,10 @@ extern "C" tl_status tl_batch_create(const tl_batch_item *items, uint32_t count,
     TL_TRY(build_batch(*B, c));
     TL_TRY(upload_batch(*B));
     TL_TRY(build_planned(*B, c));
+    // $TL_BATCH_WAVES=0 runs every array's plan launch by launch instead
+    const char *wv = getenv("TL_BATCH_WAVES");
+    B->waves = !(wv && atoi(wv) == 0);
+    if (B->waves) TL_TRY(build_waves(*B, c));
     *handle = B.release();
     return TL_OK;
 }

@@ -405,17 +405,11 @@ __device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t s
     }
 }
 
-template <typename T, int PER, int MODE = CH_FORWARD>
-__global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
-                                                 uint32_t *flags, unsigned long long *counter,
-                                                 uint32_t *err, uint64_t limit, T *stash) {
-    __shared__ uint64_t s_t;
-    for (;;) {
-        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
-        __syncthreads();
-        const uint64_t t = s_t;
-        __syncthreads();
-        if (t >= nmoves) return;
+// one chained move, t, by the whole workgroup: load, publish, wait, store
+template <typename T, int PER, int MODE>
+__device__ inline void chained_move(T *__restrict__ A, const MoveSpec &ms, uint64_t t,
+                                    uint32_t *flags, uint32_t *err, uint64_t limit, T *stash) {
+    {
         uint64_t src, dst, len;
         if constexpr (MODE != CH_FORWARD)
             move_at_spread(ms, t, src, dst, len);
@@ -446,6 +440,21 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
     }
 }
 
+template <typename T, int PER, int MODE = CH_FORWARD>
+__global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
+                                                 uint32_t *flags, unsigned long long *counter,
+                                                 uint32_t *err, uint64_t limit, T *stash) {
+    __shared__ uint64_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        __syncthreads();
+        if (t >= nmoves) return;
+        chained_move<T, PER, MODE>(A, ms, t, flags, err, limit, stash);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // one-pass rotation of A[0, L) right by c (key at p goes to p + c mod L) by
 // parallel cycle leaders.  The first w keys are stashed beforehand; lane j
@@ -457,10 +466,8 @@ __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms,
 // neighbouring keys, so a wave's accesses are contiguous; each lane keeps K
 // loads in flight.
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t L, uint64_t c,
-                                                     uint64_t w, const T *__restrict__ stash) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= w) return;
+__device__ inline void walk_lane(T *__restrict__ A, uint64_t L, uint64_t c, uint64_t w,
+                                 const T *__restrict__ stash, uint64_t j) {
     const uint64_t back = L - c;   // p - c (mod L) = p + back when p < c
     uint64_t dst = j;
     for (;;) {
@@ -490,6 +497,13 @@ __global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t
         if (stop) return;
         dst = s[K - 1];
     }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t L, uint64_t c,
+                                                     uint64_t w, const T *__restrict__ stash) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < w) walk_lane<T, K>(A, L, c, w, stash, j);
 }
 
 // ---------------------------------------------------------------------------

@@ -54,18 +54,19 @@ __device__ inline uint32_t lds_slot(uint32_t x, uint32_t n, uint32_t kind, uint3
 
 // inv = 1 applies the inverse permutation (slot -> rank instead of rank ->
 // slot): the keys go into LDS in place and come out gathered through the slots.
+// one workgroup's share: items [group*per, group*per + per) of the runs
 template <typename T>
-__global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
-                                                  uint32_t nruns, uint64_t nitems, uint32_t per,
-                                                  uint32_t n, uint32_t kind, uint32_t b,
-                                                  uint32_t rows, uint32_t magic, uint32_t inv) {
+__device__ inline void lds_body(T *__restrict__ A, const uint64_t *__restrict__ runs,
+                                uint32_t nruns, uint64_t nitems, uint32_t per, uint32_t n,
+                                uint32_t kind, uint32_t b, uint32_t rows, uint32_t magic,
+                                uint32_t inv, uint64_t group) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *s = reinterpret_cast<T *>(smem_raw);
     uint32_t *tbl = reinterpret_cast<uint32_t *>(smem_raw + (size_t)per * n * sizeof(T));
     __shared__ uint64_t s_off[64];
     BtreeShape sh;
     if (kind == LDS_BTREE) sh = btree_shape(n, b);
-    const uint64_t first = (uint64_t)blockIdx.x * per;
+    const uint64_t first = group * per;
     const uint32_t cnt = (uint32_t)min((uint64_t)per, nitems - first);
     const bool table = per > 1;
     for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) s_off[t] = run_offset(runs, nruns, first + t);
@@ -102,6 +103,14 @@ __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint6
         const uint32_t p = inv ? (table ? tbl[x] : lds_slot(x, n, kind, b, rows, sh)) : x;
         A[s_off[t] + x] = s[t * n + p];
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
+                                                  uint32_t nruns, uint64_t nitems, uint32_t per,
+                                                  uint32_t n, uint32_t kind, uint32_t b,
+                                                  uint32_t rows, uint32_t magic, uint32_t inv) {
+    lds_body<T>(A, runs, nruns, nitems, per, n, kind, b, rows, magic, inv, blockIdx.x);
 }
 
 // reverse every segment [off, off+len): REV_U mirrored pairs per thread per
@@ -197,12 +206,9 @@ __global__ __launch_bounds__(256) void k_unit_merge_runs(T *__restrict__ A, cons
 // batch read before any of it is written, and each root lands after its
 // row's last key.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
-                                                            uint32_t nruns, uint64_t r, uint64_t l,
-                                                            uint32_t inv) {
+__device__ inline void stream_body(T *__restrict__ a, uint64_t r, uint64_t l, uint32_t inv) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *roots = reinterpret_cast<T *>(smem_raw);
-    T *a = A + run_offset(runs, nruns, blockIdx.x);
     const uint64_t L1 = l + 1;
     const uint64_t total = r * L1;
     constexpr int U = 8;
@@ -270,6 +276,13 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
     for (uint64_t i = threadIdx.x; i < r; i += blockDim.x) a[i] = roots[i];
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
+                                                            uint32_t nruns, uint64_t r, uint64_t l,
+                                                            uint32_t inv) {
+    stream_body<T>(A + run_offset(runs, nruns, blockIdx.x), r, l, inv);
+}
+
 // The same gather with the layouts of its kids fused in (fuse = 1 mvEB,
 // 2 classic vEB) over a whole perfect subtree, r rows of (l, 1) followed by
 // the last leaf block: the r roots become a vEB of r keys at the front and
@@ -280,14 +293,11 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
 // the same way from the left with the slot map inverted in LDS.  Needs
 // l + 1 <= 256 * U.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sparse_stream_fused(T *__restrict__ A,
-                                                             const uint64_t *__restrict__ runs,
-                                                             uint32_t nruns, uint32_t r, uint32_t l,
-                                                             uint32_t inv, uint32_t fuse) {
+__device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t l, uint32_t inv,
+                                         uint32_t fuse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *roots = reinterpret_cast<T *>(smem_raw);
     uint32_t *slot = reinterpret_cast<uint32_t *>(smem_raw + (((size_t)r * sizeof(T) + 15) & ~15ull));
-    T *a = A + run_offset(runs, nruns, blockIdx.x);
     const bool classic = fuse == 2;
     const uint32_t L1 = l + 1;
     constexpr int U = 8;
@@ -385,6 +395,14 @@ __global__ __launch_bounds__(256) void k_sparse_stream_fused(T *__restrict__ A,
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) a[mveb_pos32(i, r, classic)] = roots[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sparse_stream_fused(T *__restrict__ A,
+                                                             const uint64_t *__restrict__ runs,
+                                                             uint32_t nruns, uint32_t r, uint32_t l,
+                                                             uint32_t inv, uint32_t fuse) {
+    stream_fused_body<T>(A + run_offset(runs, nruns, blockIdx.x), r, l, inv, fuse);
 }
 
 // ---------------------------------------------------------------------------
@@ -594,6 +612,166 @@ tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W,
                            kind, b);
     TL_LAUNCH_CHECK();
     return TL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// multi-array launches (descriptors: tl_plan.hpp)
+
+// the descriptor owning task t: the last i with pre[i] <= t
+__device__ inline uint32_t find_desc(const uint64_t *__restrict__ pre, uint32_t nd, uint64_t t) {
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint64_t MCOPY_CHUNK = 2048;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
+                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MCopy m = d[i];
+    const uint64_t lo = (blockIdx.x - pre[i]) * MCOPY_CHUNK;
+    const uint64_t hi = min(m.count, lo + MCOPY_CHUNK);
+    T *dst = (T *)m.dst;
+    const T *src = (const T *)m.src;
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) dst[k * m.dstride] = src[k * m.sstride];
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_multi_lds(const MLds *__restrict__ d,
+                                                   const uint64_t *__restrict__ pre, uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MLds m = d[i];
+    lds_body<T>((T *)m.A, (const uint64_t *)m.runs, m.nruns, m.nitems, m.per, m.n, m.kind, m.b,
+                m.rows, m.magic, m.inv, blockIdx.x - pre[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_multi_stream(const MStream *__restrict__ d,
+                                                      const uint64_t *__restrict__ pre,
+                                                      uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MStream m = d[i];
+    T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, blockIdx.x - pre[i]);
+    if (m.fuse)
+        stream_fused_body<T>(a, m.r, m.l, m.inv, m.fuse);
+    else
+        stream_body<T>(a, m.r, m.l, m.inv);
+}
+
+// moves of many chained ops, numbered op after op: a move waits only for
+// lower-numbered moves of its own op, which hold lower tickets
+template <typename T, int PER, int MODE>
+__global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict__ d,
+                                                       const uint64_t *__restrict__ pre, uint32_t nd,
+                                                       uint64_t total, unsigned long long *counter,
+                                                       uint32_t *err, uint64_t limit) {
+    __shared__ uint64_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        __syncthreads();
+        if (t >= total) return;
+        const uint32_t i = find_desc(pre, nd, t);
+        const MChain m = d[i];
+        chained_move<T, PER, MODE>((T *)m.A, m.ms, t - pre[i], (uint32_t *)m.flags, err, limit,
+                                   (T *)m.stash);
+    }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict__ d,
+                                                      const uint64_t *__restrict__ pre,
+                                                      uint32_t nd) {
+    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (t >= pre[nd]) return;
+    const uint32_t i = find_desc(pre, nd, t);
+    const MBitrev m = d[i];
+    const uint64_t g = t - pre[i];
+    if (m.levels) {
+        const uint32_t j = 63u - (uint32_t)__clzll(g + 1);
+        const uint32_t lev = m.t0 + j;
+        bitrev_tile<T, K>((T *)m.A, (int64_t)((1ull << lev) - 1), lev, g - ((1ull << j) - 1));
+    } else {
+        bitrev_tile<T, K>((T *)m.A, m.base, m.D, g);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_multi_walk(const MWalk *__restrict__ d,
+                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MWalk m = d[i];
+    const uint64_t j = (blockIdx.x - pre[i]) * 256 + threadIdx.x;
+    if (j < m.w) walk_lane<T, 4>((T *)m.A, m.L, m.c, m.w, (const T *)m.stash, j);
+}
+
+template <typename T>
+static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
+    if (!m.nd || !m.tasks) return TL_OK;
+    const uint64_t *pre = (const uint64_t *)m.pre;
+    switch (m.kind) {
+    case MK_COPY:
+        hipLaunchKernelGGL(k_multi_copy<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+                           (const MCopy *)m.desc, pre, m.nd);
+        break;
+    case MK_LDS:
+        hipLaunchKernelGGL(k_multi_lds<T>, dim3((uint32_t)m.tasks), dim3(m.threads), m.lds, s,
+                           (const MLds *)m.desc, pre, m.nd);
+        break;
+    case MK_STREAM:
+    case MK_FUSED:
+        hipLaunchKernelGGL(k_multi_stream<T>, dim3((uint32_t)m.tasks), dim3(256), m.lds, s,
+                           (const MStream *)m.desc, pre, m.nd);
+        break;
+    case MK_CHAIN: {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(m.tasks, (uint64_t)c->num_cus * 8);
+        unsigned long long *ctr = (unsigned long long *)m.counter;
+        if (m.param == CH_ROW_SPREAD)
+            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_ROW_SPREAD>), dim3(grid),
+                               dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
+                               c->err_flag, c->wait_ticks);
+        else if (m.param == CH_SPARSE_SPREAD)
+            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_SPARSE_SPREAD>), dim3(grid),
+                               dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
+                               c->err_flag, c->wait_ticks);
+        else
+            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_FORWARD>), dim3(grid),
+                               dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
+                               c->err_flag, c->wait_ticks);
+        break;
+    }
+    case MK_BITREV: {
+        const uint32_t gx = (uint32_t)std::min<uint64_t>(m.tasks, 1u << 20);
+        const uint32_t gy = (uint32_t)((m.tasks + gx - 1) / gx);
+        if (m.param == 6)
+            hipLaunchKernelGGL((k_multi_bitrev<T, 6>), dim3(gx, gy), dim3(256), 0, s,
+                               (const MBitrev *)m.desc, pre, m.nd);
+        else
+            hipLaunchKernelGGL((k_multi_bitrev<T, 5>), dim3(gx, gy), dim3(256), 0, s,
+                               (const MBitrev *)m.desc, pre, m.nd);
+        break;
+    }
+    case MK_WALK:
+        hipLaunchKernelGGL(k_multi_walk<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+                           (const MWalk *)m.desc, pre, m.nd);
+        break;
+    default: set_error("multi_launch: unknown kind"); return TL_EINVAL;
+    }
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
+tl_status multi_launch(Ctx *c, const MultiLaunch &m, hipStream_t s) {
+    return m.W == 4 ? multi_launch_t<uint32_t>(c, m, s) : multi_launch_t<uint64_t>(c, m, s);
 }
 
 static tl_status check_args(Layout lay, const void *dev_A, uint64_t n, uint64_t b, tl_dtype dt,

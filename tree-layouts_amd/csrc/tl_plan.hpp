@@ -109,6 +109,47 @@ tl_status fill_u32_launch(uint32_t *p, uint64_t count, uint32_t v, hipStream_t s
 tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W, uint32_t kind,
                            uint32_t b, uint32_t max_n, hipStream_t s);
 
+// ---- multi-array launches: one launch runs the same kind of op for many
+//      independent arrays (the uneven batch runs a plan's k-th op of every
+//      array together).  Descriptors hold absolute device addresses; `pre`
+//      is the prefix sum of each descriptor's workgroup (or move) count.
+struct MCopy {   // dst[i*dstride] = src[i*sstride], i < count
+    uint64_t dst, src, dstride, sstride, count;
+};
+struct MLds {    // one OP_LDS of one array
+    uint64_t A, runs, nitems;
+    uint32_t nruns, per, n, kind, b, rows, magic, inv;
+};
+struct MStream {   // one OP_SPARSE_STREAM of one array
+    uint64_t A, runs;
+    uint32_t nruns, r, l, inv, fuse, pad;
+};
+struct MChain {    // one OP_CHAINED of one array
+    uint64_t A, flags, stash;
+    MoveSpec ms;
+};
+struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
+    uint64_t A;
+    int64_t base;
+    uint32_t D, t0, d, levels;
+};
+struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before)
+    uint64_t A, stash, L, c, w;
+};
+
+enum MultiKind : uint32_t { MK_COPY, MK_LDS, MK_STREAM, MK_FUSED, MK_CHAIN, MK_BITREV, MK_WALK };
+
+struct MultiLaunch {
+    uint32_t kind, W, param;        // param: chained mode, or bitrev K
+    uint32_t nd;                    // descriptors
+    uint64_t desc, pre;             // device addresses of descriptors and prefix sums
+    uint64_t tasks;                 // workgroups (chained: moves)
+    uint64_t counter;               // chained: ticket (zeroed with the flags)
+    size_t lds;                     // dynamic LDS bytes
+    uint32_t threads;
+};
+tl_status multi_launch(Ctx *c, const MultiLaunch &m, hipStream_t s);
+
 // LDS kept free for kernels' static shared variables next to the dynamic area
 constexpr size_t LDS_STATIC_RESERVE = 2048;
 
