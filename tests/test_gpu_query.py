@@ -87,6 +87,74 @@ def test_query_matches_oracle(layout, b, kind, dev):
         assert not np.isin(Q[~hit], keys).any()
 
 
+BTREE_FORMS = [  # (TL_BTREE_WIDE, TL_BTREE_IDX64, TL_BTREE_QPG)
+    ("0", "0", "2"), ("1", "0", "2"), ("1", "0", "4"), ("1", "1", "2"), ("1", "1", "4")]
+
+
+@pytest.mark.parametrize("wide,idx64,qpg", BTREE_FORMS)
+@pytest.mark.parametrize("b", [16, 8, 2, 32, 64])
+@pytest.mark.parametrize("kind", ["u32", "u64", "i64"])
+def test_btree_descent_forms(wide, idx64, qpg, b, kind, dev, monkeypatch):
+    """The one-key-per-lane group descent and the 16-byte-per-lane wide
+    descent (32- and 64-bit node numbers, 2 and 4 queries per group) give the
+    oracle's answers, plain and from learned start nodes."""
+    monkeypatch.setenv("TL_BTREE_WIDE", wide)
+    monkeypatch.setenv("TL_BTREE_IDX64", idx64)
+    monkeypatch.setenv("TL_BTREE_QPG", qpg)
+    rng = np.random.default_rng(77 + b)
+    for n in [1, 5, 16, 17, 1000, 65535, 100003]:
+        keys = make_keys(kind, n, rng)
+        A = ol.permute("btree", keys.copy(), b)
+        Q = make_queries(kind, keys, 8192, rng)
+        got = gpu_answers("btree", b, kind, A, Q, dev)
+        assert np.array_equal(got, ol.search("btree", A, Q, kind, b)), (b, kind, n)
+        if n < 2:   # the model needs two keys
+            continue
+        K = to_dev(keys, kind, dev)
+        model, _ = tl_amd.learned_fit(K, 64, kind=kind)
+        Ad, Qd = to_dev(A, kind, dev), to_dev(Q, kind, dev)
+        lrn = torch.empty(Q.size, dtype=torch.int64, device=dev)
+        tl_amd.query_btree_learned(Ad, b, model, Qd, lrn, None, kind=kind)
+        torch.cuda.synchronize()
+        assert np.array_equal(lrn.cpu().numpy().view(np.uint64), got), (b, kind, n)
+
+
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+def test_btree_query_unaligned_view(kind, dev):
+    """An array that starts 8 (or 4) bytes past a 16-byte boundary is served
+    by the group descent; answers are slots of the view."""
+    rng = np.random.default_rng(9)
+    n, b = 100003, 16
+    keys = make_keys(kind, n, rng)
+    A = ol.permute("btree", keys.copy(), b)
+    Q = make_queries(kind, keys, 8192, rng)
+    st_np, st_t = STORE[kind]
+    big = torch.zeros(n + 1, dtype=st_t, device=dev)
+    big[1:] = torch.from_numpy(A.view(st_np).copy()).to(dev)
+    view = big[1:]
+    assert view.data_ptr() % 16 != 0
+    ans = torch.empty(Q.size, dtype=torch.int64, device=dev)
+    tl_amd.query_btree(view, b, to_dev(Q, kind, dev), ans, kind=kind)
+    torch.cuda.synchronize()
+    assert np.array_equal(ans.cpu().numpy().view(np.uint64), ol.search("btree", A, Q, kind, b))
+
+
+@pytest.mark.parametrize("wide", ["0", "1"])
+@pytest.mark.parametrize("b", [16, 8])
+def test_btree_query_duplicate_keys(wide, b, dev, monkeypatch):
+    """Runs of equal keys (inside and across nodes): the answer is the
+    reference's first key not smaller than q on the descent path."""
+    monkeypatch.setenv("TL_BTREE_WIDE", wide)
+    rng = np.random.default_rng(31 + b)
+    for n in [17, 1000, 100003]:
+        keys = np.sort(rng.integers(0, max(2, n // 7), size=n)).astype(np.uint64)
+        A = ol.permute("btree", keys.copy(), b)
+        Q = np.concatenate([keys[rng.integers(0, n, 4000)],
+                            rng.integers(0, n // 7 + 3, 1000).astype(np.uint64)])
+        got = gpu_answers("btree", b, "u64", A, Q, dev)
+        assert np.array_equal(got, ol.search("btree", A, Q, "u64", b)), (b, n)
+
+
 @pytest.mark.parametrize("kind", ["u32", "u64", "i32", "i64"])
 def test_binary_search_matches_oracle(kind, dev):
     """Sorted array: hit -> index, miss -> predecessor's index, below the

@@ -377,6 +377,58 @@ __global__ __launch_bounds__(256) void k_query_btree_from(const K *__restrict__ 
     }
 }
 
+// Pass 2, wide form (descent_wide): the start slots are node starts (k * b),
+// QN queries per group; the same redo from the root.
+template <typename K, int G, int QN, typename IDX>
+__global__ __launch_bounds__(256) void k_query_btree_from_wide(const K *__restrict__ A, uint64_t n,
+                                                               uint32_t b, const K *__restrict__ Q,
+                                                               uint64_t nq,
+                                                               uint64_t *__restrict__ ans,
+                                                               unsigned long long *stats) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = lane % G;
+    const uint64_t gm = group_mask<G>(lane);
+    const IDX nodes = (IDX)((n + b - 1) / b);
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint64_t trips = (nq + QN * groups - 1) / (QN * groups);   // uniform trip count
+    uint64_t q0 = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    uint32_t fb = 0;
+    for (uint64_t it = 0; it < trips; it++, q0 += QN * groups) {
+        K q[QN];
+        IDX node[QN];
+        bool act[QN], found[QN], redo[QN];
+        uint64_t st[QN], res[QN], res2[QN];
+#pragma unroll
+        for (int k = 0; k < QN; k++) {
+            const uint64_t qi = q0 + k * groups;
+            act[k] = qi < nq;
+            q[k] = act[k] ? Q[qi] : K{};
+            st[k] = act[k] ? ans[qi] : 0;
+            node[k] = (IDX)(st[k] / b);
+        }
+        descent_wide<K, QN, IDX>(A, n, b, nodes, gm, j, q, node, act, res, found);
+#pragma unroll
+        for (int k = 0; k < QN; k++) {
+            redo[k] = q0 + k * groups < nq && st[k] != 0 && !found[k];
+            act[k] = redo[k];
+            node[k] = 0;
+            if (j == 0) fb += redo[k];
+        }
+        descent_wide<K, QN, IDX>(A, n, b, nodes, gm, j, q, node, act, res2, found);
+        if (j == 0) {
+#pragma unroll
+            for (int k = 0; k < QN; k++) {
+                const uint64_t qi = q0 + k * groups;
+                if (qi < nq) ans[qi] = redo[k] ? res2[k] : res[k];
+            }
+        }
+    }
+    if (stats) {
+        for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
+        if (lane == 0 && fb) atomicAdd(&stats[0], (unsigned long long)fb);
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 template <typename K>
@@ -463,6 +515,32 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
         hipLaunchKernelGGL((k_learned_start<K, false>), dim3((uint32_t)sgrid), dim3(256), 0, s, bc,
                            hp, leaves, (const K *)q, nq, ans, stats);
     TL_LAUNCH_CHECK();
+    if (wide_ok(A, b, sizeof(K))) {
+        const uint32_t G = wide_group(b, sizeof(K));
+        const int QN = btree_qpg(2);
+        const bool narrow = wide_narrow(n, b);
+        const uint64_t want = (nq + QN * 256 / G - 1) / (QN * 256 / G);
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));
+#define TL_WL(GG, QQ, IX)                                                                          \
+    hipLaunchKernelGGL((k_query_btree_from_wide<K, GG, QQ, IX>), dim3(grid), dim3(256), 0, s,      \
+                       (const K *)A, n, (uint32_t)b, (const K *)q, nq, ans, stats)
+#define TL_W(GG)                                                                                   \
+    case GG:                                                                                       \
+        if (QN == 4) {                                                                             \
+            if (narrow) TL_WL(GG, 4, uint32_t);                                                    \
+            else TL_WL(GG, 4, uint64_t);                                                           \
+        } else {                                                                                   \
+            if (narrow) TL_WL(GG, 2, uint32_t);                                                    \
+            else TL_WL(GG, 2, uint64_t);                                                           \
+        }                                                                                          \
+        break;
+        switch (G) { TL_W(1) TL_W(2) TL_W(4) TL_W(8) TL_W(16) TL_W(32) }
+#undef TL_W
+#undef TL_WL
+        TL_LAUNCH_CHECK();
+        return TL_OK;
+    }
     const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
     const uint64_t want = (nq + 2 * 256 / G - 1) / (2 * 256 / G);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));

@@ -100,6 +100,42 @@ __global__ __launch_bounds__(256) void k_query_btree_group(const K *__restrict__
     }
 }
 
+// B-tree, wide form (descent_wide): groups of G lanes holding 16 bytes of
+// the node each, QN queries per group.
+template <typename K, int G, int QN, typename IDX>
+__global__ __launch_bounds__(256) void k_query_btree_wide(const K *__restrict__ A, uint64_t n,
+                                                          uint32_t b, const K *__restrict__ Q,
+                                                          uint64_t nq, uint64_t *__restrict__ ans) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = lane % G;
+    const uint64_t gm = group_mask<G>(lane);
+    const IDX nodes = (IDX)((n + b - 1) / b);
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint64_t trips = (nq + QN * groups - 1) / (QN * groups);   // uniform trip count
+    uint64_t q0 = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    for (uint64_t it = 0; it < trips; it++, q0 += QN * groups) {
+        K q[QN];
+        IDX node[QN];
+        bool act[QN], found[QN];
+        uint64_t res[QN];
+#pragma unroll
+        for (int k = 0; k < QN; k++) {
+            const uint64_t qi = q0 + k * groups;
+            act[k] = qi < nq;
+            q[k] = act[k] ? Q[qi] : K{};
+            node[k] = 0;
+        }
+        descent_wide<K, QN, IDX>(A, n, b, nodes, gm, j, q, node, act, res, found);
+        if (j == 0) {
+#pragma unroll
+            for (int k = 0; k < QN; k++) {
+                const uint64_t qi = q0 + k * groups;
+                if (qi < nq) ans[qi] = res[k];
+            }
+        }
+    }
+}
+
 // B-tree, lane form (b == 1 or b > 64): the reference's linear node scan
 template <typename K>
 __global__ __launch_bounds__(256) void k_query_btree_lane(const K *__restrict__ A, uint64_t n,
@@ -431,7 +467,28 @@ static void launch_binary(Ctx *c, const void *A, uint64_t n, const void *q, uint
 template <typename K>
 static void launch_btree(Ctx *c, const void *A, uint64_t n, uint64_t b, const void *q, uint64_t nq,
                          uint64_t *ans, hipStream_t s) {
-    if (b >= 2 && b <= 64) {
+    if (wide_ok(A, b, sizeof(K))) {
+        const uint32_t G = wide_group(b, sizeof(K));
+        const int QN = btree_qpg(4);
+        const bool narrow = wide_narrow(n, b);
+        const uint32_t grid = grid_for(c, nq, QN * 256 / G);
+#define TL_WL(GG, QQ, IX)                                                                          \
+    hipLaunchKernelGGL((k_query_btree_wide<K, GG, QQ, IX>), dim3(grid), dim3(256), 0, s,           \
+                       (const K *)A, n, (uint32_t)b, (const K *)q, nq, ans)
+#define TL_W(GG)                                                                                   \
+    case GG:                                                                                       \
+        if (QN == 4) {                                                                             \
+            if (narrow) TL_WL(GG, 4, uint32_t);                                                    \
+            else TL_WL(GG, 4, uint64_t);                                                           \
+        } else {                                                                                   \
+            if (narrow) TL_WL(GG, 2, uint32_t);                                                    \
+            else TL_WL(GG, 2, uint64_t);                                                           \
+        }                                                                                          \
+        break;
+        switch (G) { TL_W(1) TL_W(2) TL_W(4) TL_W(8) TL_W(16) TL_W(32) }
+#undef TL_W
+#undef TL_WL
+    } else if (b >= 2 && b <= 64) {
         const uint32_t G = b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64;
         const uint32_t grid = grid_for(c, nq, 2 * 256 / G);
         switch (G) {
