@@ -23,9 +23,11 @@ def _keys(kind, n, rng):
     return (k - k[n // 2]).astype(np.int32) if kind == "i32" else k.astype(np.uint32)
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
 @pytest.mark.parametrize("kind", ["u64", "i64", "u32", "i32"])
 @pytest.mark.parametrize("b", [16, 8, 5, 32])
-def test_learned_equals_plain(kind, b, dev):
+def test_learned_equals_plain(kind, b, fused, dev, monkeypatch):
+    monkeypatch.setenv("TL_LEARNED_FUSED", fused)
     rng = np.random.default_rng(44 + b)
     for n, leaves in [(3, 4), (100, 8), (5000, 64), (100003, 1 << 10), (1 << 20, 1 << 16),
                       (3_000_001, 1 << 16)]:

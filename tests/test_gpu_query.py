@@ -87,20 +87,23 @@ def test_query_matches_oracle(layout, b, kind, dev):
         assert not np.isin(Q[~hit], keys).any()
 
 
-BTREE_FORMS = [  # (TL_BTREE_WIDE, TL_BTREE_IDX64, TL_BTREE_QPG)
-    ("0", "0", "2"), ("1", "0", "2"), ("1", "0", "4"), ("1", "1", "2"), ("1", "1", "4")]
+BTREE_FORMS = [  # (TL_BTREE_WIDE, TL_BTREE_IDX64, TL_BTREE_QPG, TL_LEARNED_FUSED)
+    ("0", "0", "2", "1"), ("1", "0", "2", "1"), ("1", "0", "2", "0"), ("1", "0", "4", "0"),
+    ("1", "1", "2", "1"), ("1", "1", "4", "1")]
 
 
-@pytest.mark.parametrize("wide,idx64,qpg", BTREE_FORMS)
+@pytest.mark.parametrize("wide,idx64,qpg,fused", BTREE_FORMS)
 @pytest.mark.parametrize("b", [16, 8, 2, 32, 64])
 @pytest.mark.parametrize("kind", ["u32", "u64", "i64"])
-def test_btree_descent_forms(wide, idx64, qpg, b, kind, dev, monkeypatch):
+def test_btree_descent_forms(wide, idx64, qpg, fused, b, kind, dev, monkeypatch):
     """The one-key-per-lane group descent and the 16-byte-per-lane wide
     descent (32- and 64-bit node numbers, 2 and 4 queries per group) give the
-    oracle's answers, plain and from learned start nodes."""
+    oracle's answers, plain and from learned start nodes (two passes, or the
+    one-pass learned kernel)."""
     monkeypatch.setenv("TL_BTREE_WIDE", wide)
     monkeypatch.setenv("TL_BTREE_IDX64", idx64)
     monkeypatch.setenv("TL_BTREE_QPG", qpg)
+    monkeypatch.setenv("TL_LEARNED_FUSED", fused)
     rng = np.random.default_rng(77 + b)
     for n in [1, 5, 16, 17, 1000, 65535, 100003]:
         keys = make_keys(kind, n, rng)

@@ -179,6 +179,14 @@ inline bool wide_narrow(uint64_t n, uint64_t b) {
     return nodes < (1ull << 32) / (b + 1);
 }
 
+// the learned query in one launch (start nodes and descents) where the wide
+// form applies with 32-bit node numbers and n < 2^32; $TL_LEARNED_FUSED=0
+// keeps the two passes (start slots through the answer array)
+inline bool learned_fused() {
+    const char *e = getenv("TL_LEARNED_FUSED");
+    return !(e && atoi(e) == 0);
+}
+
 // queries per group of the wide descents: $TL_BTREE_QPG (2 or 4) when set,
 // else def.  Measured at n = 2^30, b = 16, 8-byte keys: the root descent is
 // fastest with 4 (12.7 ms for 2^27 hits, 13.6 ms with 2), the learned descent

@@ -267,6 +267,34 @@ __device__ inline uint64_t start_slot(uint64_t lo, uint64_t hi, const BtreeConst
     return (nodestart[h] + div_pw<FAST>(plo, h, pw, mag)) * c.b;
 }
 
+// the start slot of one query key divided by b (a node number; every start
+// slot is a node start), 0 (the root) when the model has no usable window
+template <typename K, bool FAST>
+__device__ inline uint64_t start_node(K key, const LearnedHeader &h,
+                                      const LeafModel *__restrict__ leaves, const BtreeConst &c,
+                                      const uint64_t *pw, const uint64_t *mag,
+                                      const uint64_t *nodestart, uint64_t nint_nodes,
+                                      bool &fallback) {
+    const uint64_t u = ukey<K>(key);
+    const LeafModel m = leaves[leaf_of(u, h)];
+    fallback = true;
+    if (!m.count) return 0;
+    const int64_t pred = predict(u, m);
+    int64_t lo = pred - (int64_t)m.err, hi = pred + (int64_t)m.err;
+    lo = max(lo, (int64_t)m.base_rank);
+    hi = min(hi, (int64_t)(m.base_rank + m.count - 1));
+    if (lo > hi) return 0;
+    fallback = false;
+    uint64_t plo, qlo, phi, qhi;
+    bool llo, lhi;
+    rank_place<FAST>((uint64_t)lo, c, pw, mag, plo, qlo, llo);
+    rank_place<FAST>((uint64_t)hi, c, pw, mag, phi, qhi, lhi);
+    if (llo && lhi && plo == phi) return nint_nodes + plo;   // one leaf node
+    uint32_t ht = 1;
+    while (ht < c.D && div_pw<FAST>(plo, ht, pw, mag) != div_pw<FAST>(qhi, ht, pw, mag)) ht++;
+    return nodestart[ht] + div_pw<FAST>(plo, ht, pw, mag);
+}
+
 constexpr int START_QPL = 4;   // start-pass queries per lane
 
 // Pass 1: the start slot of every query, one lane per query, written into the
@@ -429,6 +457,86 @@ __global__ __launch_bounds__(256) void k_query_btree_from_wide(const K *__restri
     }
 }
 
+// One pass (wide form, n < 2^32, 32-bit node numbers): a group of G lanes
+// takes G consecutive queries per trip; lane j computes the start node of
+// query j (the group's model loads in flight together, no start slots through
+// HBM), then the group descends the G queries QN at a time, keys and start
+// nodes broadcast by lane shuffles, and lane j stores query j's answer.  The
+// fallback count and the redo from the root are those of the two passes.
+template <typename K, int G, int QN>
+__global__ __launch_bounds__(256) void k_query_btree_learned_fused(
+    BtreeConst bc, const LearnedHeader *__restrict__ hp, const LeafModel *__restrict__ leaves,
+    const K *__restrict__ A, const K *__restrict__ Q, uint64_t nq, uint64_t *__restrict__ ans,
+    unsigned long long *stats) {
+    static_assert(G >= QN && G % QN == 0, "a group descends its queries QN at a time");
+    __shared__ uint64_t s_pw[41], s_mag[41], s_ns[41];
+    for (uint32_t t = threadIdx.x; t < 41; t += blockDim.x) {
+        s_pw[t] = bc.pw[t];
+        s_mag[t] = bc.mag[t];
+        s_ns[t] = bc.nodestart[t];
+    }
+    __syncthreads();
+    const LearnedHeader h = *hp;
+    const uint64_t n = bc.n;
+    const uint32_t b = (uint32_t)bc.b;
+    const uint64_t nint_nodes = bc.nint / bc.b;
+    const uint32_t nodes = (uint32_t)((n + b - 1) / b);
+    const uint32_t lane = threadIdx.x & 63, j = lane % G, gbase = lane - j;
+    const uint64_t gm = group_mask<G>(lane);
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint64_t gid = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    const uint64_t trips = (nq + groups * G - 1) / (groups * G);   // uniform trip count
+    uint32_t fb = 0;
+    for (uint64_t it = 0; it < trips; it++) {
+        const uint64_t qi = (it * groups + gid) * G + j;
+        const bool valid = qi < nq;
+        const K myq = valid ? Q[qi] : K{};
+        uint32_t mystart = 0;
+        if (valid) {
+            bool fall;
+            mystart = (uint32_t)start_node<K, true>(myq, h, leaves, bc, s_pw, s_mag, s_ns,
+                                                    nint_nodes, fall);
+            fb += fall;
+        }
+        uint64_t myres = n;
+#pragma unroll 1
+        for (uint32_t r = 0; r < G / QN; r++) {
+            K q[QN];
+            uint32_t node[QN], st[QN];
+            bool act[QN], found[QN], redo[QN];
+            uint64_t res[QN], res2[QN];
+#pragma unroll
+            for (int k = 0; k < QN; k++) {
+                const int src = (int)(gbase + r * QN + k);
+                q[k] = __shfl(myq, src);
+                st[k] = (uint32_t)__shfl((int)mystart, src);
+                act[k] = __shfl((int)valid, src) != 0;
+                node[k] = st[k];
+            }
+            descent_wide<K, QN, uint32_t>(A, n, b, nodes, gm, j, q, node, act, res, found);
+#pragma unroll
+            for (int k = 0; k < QN; k++) {
+                const bool v = __shfl((int)valid, (int)(gbase + r * QN + k)) != 0;
+                redo[k] = v && st[k] != 0 && !found[k];
+                act[k] = redo[k];
+                node[k] = 0;
+            }
+            descent_wide<K, QN, uint32_t>(A, n, b, nodes, gm, j, q, node, act, res2, found);
+#pragma unroll
+            for (int k = 0; k < QN; k++)
+                if (j == r * QN + k) {
+                    myres = redo[k] ? res2[k] : res[k];
+                    fb += redo[k];
+                }
+        }
+        if (valid) ans[qi] = myres;
+    }
+    if (stats) {
+        for (int o = 32; o > 0; o >>= 1) fb += __shfl_xor(fb, o);
+        if (lane == 0 && fb) atomicAdd(&stats[0], (unsigned long long)fb);
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 template <typename K>
@@ -505,6 +613,24 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
     const bool fast = n < (1ull << 32);
     const LearnedHeader *hp = (const LearnedHeader *)model;
     const LeafModel *leaves = (const LeafModel *)((const char *)model + sizeof(LearnedHeader));
+    if (learned_fused() && fast && wide_ok(A, b, sizeof(K)) && wide_narrow(n, b) &&
+        wide_group(b, sizeof(K)) >= 2) {
+        const uint32_t G = wide_group(b, sizeof(K));
+        const uint64_t want = (nq + 256 - 1) / 256;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));
+        switch (G) {
+#define TL_F(GG)                                                                                   \
+    case GG:                                                                                       \
+        hipLaunchKernelGGL((k_query_btree_learned_fused<K, GG, 2>), dim3(grid), dim3(256), 0, s,  \
+                           bc, hp, leaves, (const K *)A, (const K *)q, nq, ans, stats);            \
+        break;
+            TL_F(2) TL_F(4) TL_F(8) TL_F(16) TL_F(32)
+#undef TL_F
+        }
+        TL_LAUNCH_CHECK();
+        return TL_OK;
+    }
     const uint64_t sgrid = std::max<uint64_t>(
         1, std::min<uint64_t>((nq + 256 * START_QPL - 1) / (256 * START_QPL),
                               (uint64_t)c->num_cus * 16));
