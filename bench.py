@@ -409,18 +409,21 @@ def main():
         torch.equal(A[ans.clamp(0, n - 1)], Q))
     log(f"[bench] plain query {q_ms:.3f} ms ({nq / q_ms / 1e3:.1f} Mq/s) ok={plain_ok}")
 
-    # ---- the comparator: binary search of the sorted keys, same queries
+    # ---- the comparator: binary search of the sorted keys, same queries (its
+    # own answer buffer: the plain answers are what the learned ones must equal)
+    bans = torch.empty_like(ans)
     for _ in range(max(1, args.warmup)):
-        tl_amd.query_binary(keys, Q, ans, kind="u64", timed=False)
+        tl_amd.query_binary(keys, Q, bans, kind="u64", timed=False)
     torch.cuda.synchronize()
     ev0.record(stream)
     for _ in range(args.steps):
-        tl_amd.query_binary(keys, Q, ans, kind="u64", timed=False)
+        tl_amd.query_binary(keys, Q, bans, kind="u64", timed=False)
     ev1.record(stream)
     torch.cuda.synchronize()
     bs_ms = ev0.elapsed_time(ev1) / args.steps
-    bs_ok = bool(((ans >= 0) & (ans < n)).all().item()) and bool(
-        torch.equal(keys[ans.clamp(0, n - 1)], Q))
+    bs_ok = bool(((bans >= 0) & (bans < n)).all().item()) and bool(
+        torch.equal(keys[bans.clamp(0, n - 1)], Q))
+    del bans
     log(f"[bench] binary search {bs_ms:.3f} ms ({nq / bs_ms / 1e3:.1f} Mq/s) ok={bs_ok}")
 
     learned = None
