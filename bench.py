@@ -339,7 +339,7 @@ def main():
     ap.add_argument("--b", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--log2leaves", type=int, default=18, help="learned model leaves (log2)")
+    ap.add_argument("--log2leaves", type=int, default=20, help="learned model leaves (log2)")
     ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb"],
                     default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
