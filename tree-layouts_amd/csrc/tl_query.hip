@@ -265,29 +265,57 @@ __global__ __launch_bounds__(256) void k_query_binary(const K *__restrict__ A, u
 
 // ---------------------------------------------------------------------------
 // mvEB: one lane per query with the Brodal-Fagerberg-Jacob recurrence
-// pos[t] = pos[D[t]] + R[t] + (bfs & R[t]) * L[t]; pos[] lives in LDS as
-// [depth][lane] (32-bit when the layout has fewer than 2^32 keys) and the
-// table set is staged in LDS.  Running out of depth d_j in structure j means
-// the key is absent (the reference reads table[d] there).
+// pos[t] = pos[D[t]] + R[t] + (bfs & R[t]) * L[t].  Running out of depth d_j in
+// structure j means the key is absent (the reference reads table[d] there).
+//
+// pos[] is kept in VEB_SLOTS slots instead of one entry per depth: pos[x] is
+// only read while the descent is inside the recursion frame rooted at depth
+// x, frames nest, and the frame rooted at t sits one level below the frame
+// rooted at D[t].  So slot(t) = slot(D[t]) + 1 (slot(0) = 0, the root) gives
+// every live pos[] value its own slot: entry t reads slot(t) - 1 and writes
+// slot(t).  The recursion depth of a split of d <= 64 levels is at most 7
+// (checked for both splits on the CPU), so 8 slots suffice.  The slots live
+// in LDS as [slot][query k][lane], small enough for QPL queries per lane to
+// be in flight at full occupancy; the first VEB_TOP keys (the top frames of
+// the layout) are staged in LDS as well.
+constexpr uint32_t VEB_SLOTS = 8;
+constexpr uint32_t VEB_TOP = 2047;
+
 struct TblEnt {
     uint64_t L, R;
-    uint32_t D, pad;
+    uint32_t D, slot;
 };
 
-template <typename K, typename PT>
+// LDS layout of k_query_mveb: tables, idx, tidx, the top keys, the slots
+struct VebSmem {
+    size_t top, pos, bytes;
+};
+__host__ __device__ inline VebSmem veb_smem(uint32_t table_size, uint32_t ntop, size_t kw, size_t pw,
+                                            int qpl) {
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    VebSmem m;
+    m.top = al(table_size * sizeof(TblEnt) + 64 * 8 + 64 * 4);
+    m.pos = al(m.top + ntop * kw);
+    m.bytes = m.pos + (size_t)VEB_SLOTS * qpl * 256 * pw;
+    return m;
+}
+
+template <typename K, typename PT, int QPL>
 __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uint64_t n,
                                                     const tl_veb_table *__restrict__ tables,
                                                     uint32_t table_size,
                                                     const uint64_t *__restrict__ idx,
                                                     const uint32_t *__restrict__ tidx,
-                                                    uint32_t ntables, uint32_t dmax,
+                                                    uint32_t ntables, uint32_t ntop,
                                                     const K *__restrict__ Q, uint64_t nq,
                                                     uint64_t *__restrict__ ans) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const VebSmem lay = veb_smem(table_size, ntop, sizeof(K), sizeof(PT), QPL);
     TblEnt *tb = reinterpret_cast<TblEnt *>(smem_raw);
     uint64_t *s_idx = reinterpret_cast<uint64_t *>(tb + table_size);
     uint32_t *s_tidx = reinterpret_cast<uint32_t *>(s_idx + 64);
-    PT *pos = reinterpret_cast<PT *>(s_tidx + 64);
+    K *top = reinterpret_cast<K *>(smem_raw + lay.top);
+    PT *pos = reinterpret_cast<PT *>(smem_raw + lay.pos);
     for (uint32_t i = threadIdx.x; i < table_size; i += blockDim.x) {
         tb[i].L = tables[i].L;
         tb[i].R = tables[i].R;
@@ -297,44 +325,120 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
         s_idx[i] = idx ? idx[i] : n;
         s_tidx[i] = tidx ? tidx[i] : 0;
     }
+    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
     __syncthreads();
-    const uint32_t me = threadIdx.x, W = blockDim.x;
-    for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < nq;
-         qi += (uint64_t)gridDim.x * blockDim.x) {
-        const K q = Q[qi];
-        uint64_t res = n, offset = 0;
-        for (uint32_t j = 0; j < ntables; j++) {
-            const TblEnt *t = tb + s_tidx[j];
-            const uint32_t dj = floor_log2_64(n - offset) + 1;
-            const uint64_t end = s_idx[j];
-            uint64_t bfs = 1, index = 0;
-            uint32_t cd = 0;
-            pos[me] = 1;
-            bool done = false;
-            while (index + offset < end) {
-                const K cur = A[offset + index];
-                if (q == cur) {
-                    res = offset + index;
-                    done = true;
-                    break;
-                }
-                bfs = 2 * bfs + (q > cur);
-                cd++;
-                if (cd >= dj) {
-                    done = true;
-                    break;
-                }
-                const TblEnt e = t[cd];
-                const uint64_t p = (uint64_t)pos[e.D * W + me] + e.R + (bfs & e.R) * e.L;
-                pos[cd * W + me] = (PT)p;
-                index = p - 1;
-            }
-            if (done) break;
-            offset = end;
+    // slot of every entry: the length of its chain t -> D[t] -> .. -> 0
+    for (uint32_t i = threadIdx.x; i < table_size; i += blockDim.x) {
+        uint32_t base = 0;
+        for (uint32_t j = 0; j < ntables; j++)
+            if (s_tidx[j] <= i) base = max(base, s_tidx[j]);
+        uint32_t x = i - base, sl = 0;
+        for (uint32_t it = 0; x != 0 && it < 64; it++) {
+            const uint32_t nx = tb[base + x].D;
+            if (nx >= x) break;   // not a valid chain (a table entry never used)
+            x = nx;
+            sl++;
         }
-        ans[qi] = res;
+        tb[i].slot = min(max(sl, 1u), VEB_SLOTS - 1);   // entry 0 (the root) is never read
     }
-    (void)dmax;
+    __syncthreads();
+    const uint32_t W = blockDim.x, me = threadIdx.x;
+    auto P = [&](uint32_t slot, int k) -> PT & { return pos[(slot * QPL + k) * W + me]; };
+    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
+         q0 += all * QPL) {
+        K q[QPL];
+        uint64_t res[QPL], offset[QPL], end[QPL], bfs[QPL], index[QPL];
+        uint32_t j[QPL], cd[QPL], dj[QPL], tbase[QPL];
+        bool act[QPL];
+        // structure j of query k: from its root, or the chain's end
+        auto enter = [&](int k) {
+            while (act[k]) {
+                if (j[k] >= ntables) {
+                    act[k] = false;
+                    break;
+                }
+                end[k] = s_idx[j[k]];
+                if (offset[k] < end[k]) {
+                    tbase[k] = s_tidx[j[k]];
+                    dj[k] = floor_log2_64(n - offset[k]) + 1;
+                    bfs[k] = 1;
+                    index[k] = 0;
+                    cd[k] = 0;
+                    P(0, k) = 1;
+                    break;
+                }
+                offset[k] = end[k];
+                j[k]++;
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            act[k] = qi < nq;
+            q[k] = act[k] ? Q[qi] : K{};
+            res[k] = n;
+            offset[k] = 0;
+            j[k] = 0;
+            enter(k);
+        }
+        bool any = true;
+        while (__any(any)) {
+            K cur[QPL];
+#pragma unroll
+            for (int k = 0; k < QPL; k++) {
+                cur[k] = K{};
+                if (act[k]) {
+                    const uint64_t at = offset[k] + index[k];
+                    cur[k] = at < ntop ? top[at] : A[at];
+                }
+            }
+            any = false;
+#pragma unroll
+            for (int k = 0; k < QPL; k++) {
+                if (act[k]) {
+                    if (q[k] == cur[k]) {
+                        res[k] = offset[k] + index[k];
+                        act[k] = false;
+                    } else {
+                        bfs[k] = 2 * bfs[k] + (q[k] > cur[k]);
+                        cd[k]++;
+                        if (cd[k] >= dj[k]) {
+                            act[k] = false;
+                        } else {
+                            const TblEnt &e = tb[tbase[k] + cd[k]];
+                            const uint64_t p =
+                                (uint64_t)P(e.slot - 1, k) + e.R + (bfs[k] & e.R) * e.L;
+                            P(e.slot, k) = (PT)p;
+                            index[k] = p - 1;
+                            if (index[k] + offset[k] >= end[k]) {
+                                offset[k] = end[k];
+                                j[k]++;
+                                enter(k);
+                            }
+                        }
+                    }
+                }
+                any |= act[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QPL; k++) {
+            const uint64_t qi = q0 + k * all;
+            if (qi < nq) ans[qi] = res[k];
+        }
+    }
+}
+
+// mvEB query shape: $TL_VEB_QPL (1, 2, 4) queries per lane and $TL_VEB_TOP
+// (0: no staged top keys) when set
+inline void veb_shape(int &qpl, uint32_t &topmax) {
+    const char *e = getenv("TL_VEB_QPL");
+    const char *f = getenv("TL_VEB_TOP");
+    qpl = 2;
+    topmax = VEB_TOP;
+    if (e && (atoi(e) == 1 || atoi(e) == 2 || atoi(e) == 4)) qpl = atoi(e);
+    if (f && atoi(f) == 0) topmax = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -510,19 +614,31 @@ template <typename K>
 static void launch_mveb(Ctx *c, const void *A, uint64_t n, const tl_veb_table *tables,
                         uint32_t table_size, const uint64_t *idx, const uint32_t *tidx, uint32_t nt,
                         const void *q, uint64_t nq, uint64_t *ans, hipStream_t s) {
-    const uint32_t dmax = floor_log2_64(n ? n : 1) + 1;
+    int qpl;
+    uint32_t topmax;
+    veb_shape(qpl, topmax);
     const bool narrow = n < (1ull << 32);
-    const size_t head = table_size * sizeof(TblEnt) + 64 * 8 + 64 * 4;
-    const size_t bytes = head + (size_t)(dmax + 1) * 256 * (narrow ? 4 : 8);
-    const uint32_t grid = grid_for(c, nq, 256);
-    if (narrow)
-        hipLaunchKernelGGL((k_query_mveb<K, uint32_t>), dim3(grid), dim3(256), bytes, s,
-                           (const K *)A, n, tables, table_size, idx, tidx, nt, dmax, (const K *)q,
-                           nq, ans);
-    else
-        hipLaunchKernelGGL((k_query_mveb<K, uint64_t>), dim3(grid), dim3(256), bytes, s,
-                           (const K *)A, n, tables, table_size, idx, tidx, nt, dmax, (const K *)q,
-                           nq, ans);
+    uint32_t ntop = (uint32_t)std::min<uint64_t>(n, topmax);
+    size_t bytes = veb_smem(table_size, ntop, sizeof(K), narrow ? 4 : 8, qpl).bytes;
+    if (bytes > 64 * 1024) {   // a long table chain: no staged top keys
+        ntop = 0;
+        bytes = veb_smem(table_size, ntop, sizeof(K), narrow ? 4 : 8, qpl).bytes;
+    }
+    const uint32_t grid = grid_for(c, nq, 256 * qpl);
+#define TL_V(QQ)                                                                                   \
+    if (qpl == QQ) {                                                                               \
+        if (narrow)                                                                                \
+            hipLaunchKernelGGL((k_query_mveb<K, uint32_t, QQ>), dim3(grid), dim3(256), bytes, s,   \
+                               (const K *)A, n, tables, table_size, idx, tidx, nt, ntop,           \
+                               (const K *)q, nq, ans);                                             \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_query_mveb<K, uint64_t, QQ>), dim3(grid), dim3(256), bytes, s,   \
+                               (const K *)A, n, tables, table_size, idx, tidx, nt, ntop,           \
+                               (const K *)q, nq, ans);                                             \
+        return;                                                                                    \
+    }
+    TL_V(1) TL_V(2) TL_V(4)
+#undef TL_V
 }
 
 #define TL_DISPATCH(dt, FN, ...)                                                                 \
