@@ -100,6 +100,71 @@ def cpu_baseline(keys_np, b, nq_sample, rng):
             "cores": cores, "n": n, "nq": nq_sample}
 
 
+def run_batch_permutes(args, dev):
+    """The uneven batch's permutations alone (no queries), with the arrays of
+    the batch workload (same sizes, dtypes and layouts) laid out as slices of
+    one int32 and one int64 buffer, restored by one copy each between
+    replays, so that a profiled run makes few dispatches besides the batch's
+    own.  Starts with four calibration copies of 2^27 int64 keys (1 GiB read,
+    1 GiB written) that scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    import tl_amd
+    rng = np.random.default_rng(5)
+    count = args.batch_arrays
+    sizes = np.floor(2.0 ** rng.uniform(8, 22, size=count)).astype(np.int64)
+    kinds = np.where(rng.random(count) < 0.5, "i32", "u64")
+    layouts = ["bst", "btree", "mveb"]
+    pad = lambda x: (x + 63) // 64 * 64
+    n32 = sum(pad(int(z)) for z, k in zip(sizes, kinds) if k == "i32")
+    n64 = sum(pad(int(z)) for z, k in zip(sizes, kinds) if k == "u64")
+    cal_a = torch.arange(1 << 27, device=dev, dtype=torch.int64)
+    cal_b = torch.empty_like(cal_a)
+    for _ in range(4):
+        cal_b.copy_(cal_a)
+    del cal_a, cal_b
+    b32 = torch.arange(n32, device=dev, dtype=torch.int32)
+    b64 = torch.arange(n64, device=dev, dtype=torch.int64)
+    p32, p64 = b32.clone(), b64.clone()
+    arrays, o32, o64, keys_total, perm_bytes = [], 0, 0, 0, 0.0
+    for i in range(count):
+        n, kind = int(sizes[i]), str(kinds[i])
+        if kind == "i32":
+            A = b32[o32:o32 + n]
+            o32 += pad(n)
+        else:
+            A = b64[o64:o64 + n]
+            o64 += pad(n)
+        keys_total += n
+        perm_bytes += 2.0 * n * (4 if kind == "i32" else 8)
+        arrays.append({"A": A, "kind": kind, "layout": layouts[i % 3], "b": 8, "Q": None,
+                       "ans": None})
+    P = tl_amd.Batch(arrays, lds_bytes=args.batch_lds)
+    log(f"[bench] batch permutes: {count} arrays, {keys_total} keys")
+
+    def restore():
+        b32.copy_(p32)
+        b64.copy_(p64)
+        torch.cuda.synchronize()
+
+    times = []
+    for step in range(args.warmup + args.steps):
+        restore()
+        t = P.run(graph=True)
+        if step >= args.warmup:
+            times.append(t)
+        log(f"[bench] batch permute replay {step}: {t:.3f} ms")
+    P.close()
+    ms = float(np.mean(times))
+    traffic = measured_traffic("batch_permutes_traffic.txt")
+    out = {"workload": "uneven batch, permutations only", "ms": round(ms, 3),
+           "Gkeys_s": round(keys_total / (ms * 1e-3) / 1e9, 3),
+           "algorithmic_frac": round(perm_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "traffic": None if traffic is None else round(traffic),
+           "traffic_frac": None if traffic is None
+           else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args, dev):
     """Uneven batch: 4096 sorted arrays, sizes floor(2^U), U ~ U[8, 22]
     (seed 5), int32 / uint64 keys 50/50, layouts BST / B-tree b=8 / mvEB
@@ -200,6 +265,9 @@ def run_batch(args, dev):
     log(f"[bench] permutes alone: {perm_ms:.2f} ms")
     total_bytes = perm_bytes + query_bytes
     achieved = total_bytes / (ms * 1e-3) / 1e9
+    # HBM bytes of one permutations-only replay, from the committed PMC summary
+    perm_traffic = measured_traffic("batch_permutes_traffic.txt")
+    perm_traffic = None if perm_traffic is None else round(perm_traffic)
     out = {
         "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
         "value": round(keys_total / (ms * 1e-3) / 1e9, 3), "unit": "Gkeys/s",
@@ -215,7 +283,10 @@ def run_batch(args, dev):
         "queries": {"Mq_s": round(4 * keys_total / (ms * 1e-3) / 1e6, 1)},
         "permutes_alone": {"ms": round(perm_ms, 3),
                            "Gkeys_s": round(keys_total / (perm_ms * 1e-3) / 1e9, 3),
-                           "hbm_frac": round(perm_bytes / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                           "hbm_frac": round(perm_bytes / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "traffic": perm_traffic,
+                           "traffic_frac": None if perm_traffic is None else round(
+                               perm_traffic / (perm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "checks": {"sample_arrays": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -343,6 +414,8 @@ def main():
     ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb"],
                     default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--batch-perm-only", action="store_true",
+                    help="time the batch's permutations alone (few other dispatches)")
     ap.add_argument("--batch-lds", type=int, default=0,
                     help="LDS budget of the batch's plans in bytes (0: the library default)")
     args = ap.parse_args()
@@ -355,6 +428,8 @@ def main():
     import tl_amd
     dev = torch.device("cuda:0")
     if args.workload == "batch":
+        if args.batch_perm_only:
+            return run_batch_permutes(args, dev)
         return run_batch(args, dev)
     if args.workload in ("bst", "mveb", "veb"):
         return run_layout(args, dev, args.workload)
