@@ -914,6 +914,14 @@ static tl_status build_planned(Batch &B, Ctx *c) {
 // to a different array, so the launches of a wave are independent; a
 // rotation's stash copy runs with the wave's copies and its lane walk after
 // them.  Reversals and unit merges (rare in batches) run as single ops.
+// threads per workgroup of the batch's fused streamed gathers:
+// $TL_STREAM_THREADS (256, 512 or 1024), else 1024
+static uint32_t stream_threads() {
+    const char *e = getenv("TL_STREAM_THREADS");
+    const int v = e ? atoi(e) : 1024;
+    return (v == 256 || v == 512) ? (uint32_t)v : 1024u;
+}
+
 static tl_status build_waves(Batch &B, Ctx *c) {
     const unsigned char *pbase = (const unsigned char *)B.d_pdata;
     // a stash of its own for every array: their ops run together
@@ -925,8 +933,6 @@ static tl_status build_waves(Batch &B, Ctx *c) {
     }
     TL_HIP(hipMalloc(&B.d_wstash, std::max<uint64_t>(wst, 256)));
     const uint64_t sbase = (uint64_t)B.d_wstash;
-    size_t maxops = 0;
-    for (const LargeItem &L : B.large) maxops = std::max(maxops, L.plan.ops.size());
     struct Group {
         MultiLaunch m{};
         std::vector<unsigned char> desc;
@@ -945,13 +951,27 @@ static tl_status build_waves(Batch &B, Ctx *c) {
     };
     std::vector<Pending> pend;
     uint32_t nchain = 0;
-    for (size_t k = 0; k < maxops; k++) {
-        std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> ga, gb;
+    // Greedy schedule: each launch takes, among the next pending ops of all
+    // arrays, those of one launch kind -- the kind with the most workgroups
+    // ready -- so ops of one kind at different positions of different plans
+    // share a launch instead of spreading over many thin ones.  Every
+    // array's ops stay in plan order, and a launch holds at most one op per
+    // array.  A rotation is two steps: its stash copy, then its lane walk.
+    std::vector<size_t> nk(B.large.size(), 0);       // next op of every array
+    std::vector<uint8_t> npart(B.large.size(), 0);   // 1: a rotation's walk is next
+    size_t left = 0;
+    for (const LargeItem &L : B.large) left += L.plan.ops.size();
+    using GKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>;
+    while (left) {
+        std::map<GKey, Group> ga;
+        std::map<GKey, std::vector<size_t>> who;
         std::vector<Pending> singles;
-        auto add = [](std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> &g,
-                      uint32_t kind, uint32_t W, uint32_t param, uint32_t threads, const void *d,
-                      size_t dsize, uint64_t tasks, size_t lds) {
-            Group &G = g[std::make_tuple(kind, W, param, threads)];
+        size_t cur = 0;
+        auto add = [&who, &cur](std::map<GKey, Group> &g, uint32_t kind, uint32_t W,
+                                uint32_t param, uint32_t threads, const void *d, size_t dsize,
+                                uint64_t tasks, size_t lds) {
+            const GKey key = std::make_tuple(kind, W, param, threads);
+            Group &G = g[key];
             G.m.kind = kind;
             G.m.W = W;
             G.m.param = param;
@@ -961,11 +981,14 @@ static tl_status build_waves(Batch &B, Ctx *c) {
             G.desc.resize(at + dsize);
             memcpy(G.desc.data() + at, d, dsize);
             G.pre.push_back(G.pre.back() + tasks);
+            who[key].push_back(cur);
         };
         for (size_t li = 0; li < B.large.size(); li++) {
             const LargeItem &L = B.large[li];
+            const size_t k = nk[li];
             if (k >= L.plan.ops.size()) continue;
             const Op &op = L.plan.ops[k];
+            cur = li;
             const uint32_t W = L.W;
             const uint64_t A = (uint64_t)B.items[L.item].A;
             const uint64_t runs = (uint64_t)(pbase + L.at + op.runs_at * 8);
@@ -981,11 +1004,14 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[2] + 2047) / 2048, 0);
                 break;
             }
-            case OP_ROTATE: {
-                const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3]};
-                add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
-                const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3]};
-                add(gb, MK_WALK, W, 0, 256, &w, sizeof w, (op.p[3] + 255) / 256, 0);
+            case OP_ROTATE: {   // the stash copy, then (a later launch) the lane walk
+                if (!npart[li]) {
+                    const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3]};
+                    add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
+                } else {
+                    const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3]};
+                    add(ga, MK_WALK, W, 0, 256, &w, sizeof w, (op.p[3] + 255) / 256, 0);
+                }
                 break;
             }
             case OP_LDS: {
@@ -1003,8 +1029,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 const MStream d{A, runs, (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
                                 (uint32_t)op.p[2], (uint32_t)op.p[3], 0};
                 const bool fused = op.p[3] != 0;
-                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, 256, &d, sizeof d, op.nitems,
-                    fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
+                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, fused ? stream_threads() : 256, &d,
+                    sizeof d, op.nitems, fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
                 break;
             }
             case OP_CHAINED: {
@@ -1041,22 +1067,40 @@ static tl_status build_waves(Batch &B, Ctx *c) {
             }
             }
         }
-        auto flush = [&](std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, Group> &g) {
-            for (auto &kv : g) {
-                Group &G = kv.second;
-                Pending p;
-                p.st.m = G.m;
-                p.st.m.nd = (uint32_t)(G.pre.size() - 1);
-                p.st.m.tasks = G.pre.back();
-                p.desc_at = put(G.desc.data(), G.desc.size());
-                p.pre_at = put(G.pre.data(), G.pre.size() * 8);
-                if (G.m.kind == MK_CHAIN) p.st.m.counter = nchain++;
-                pend.push_back(p);
+        auto advance = [&](size_t li) {
+            const Op &op = B.large[li].plan.ops[nk[li]];
+            if (op.type == OP_ROTATE && !npart[li]) {
+                npart[li] = 1;
+                return;
             }
+            npart[li] = 0;
+            nk[li]++;
+            left--;
         };
-        flush(ga);
-        for (Pending &p : singles) pend.push_back(p);
-        flush(gb);
+        if (!singles.empty()) {   // ops without a multi-array kernel go alone, first
+            for (Pending &p : singles) {
+                pend.push_back(p);
+                advance(p.st.large);
+            }
+            continue;
+        }
+        if (ga.empty()) {   // every pending op maps to a launch kind or runs alone
+            set_error("tl_batch: an op with no launch kind");
+            return TL_EINVAL;
+        }
+        auto best = ga.begin();
+        for (auto it = ga.begin(); it != ga.end(); ++it)
+            if (it->second.pre.back() > best->second.pre.back()) best = it;
+        Group &G = best->second;
+        Pending p;
+        p.st.m = G.m;
+        p.st.m.nd = (uint32_t)(G.pre.size() - 1);
+        p.st.m.tasks = G.pre.back();
+        p.desc_at = put(G.desc.data(), G.desc.size());
+        p.pre_at = put(G.pre.data(), G.pre.size() * 8);
+        if (G.m.kind == MK_CHAIN) p.st.m.counter = nchain++;
+        pend.push_back(p);
+        for (size_t li : who[best->first]) advance(li);
     }
     TL_HIP(hipMalloc(&B.d_wdata, std::max<size_t>(wdata.size(), 64)));
     if (!wdata.empty())

@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_sparse_stream_runs(T *__restrict__ A, c
 // its keys are scattered) lies right of the batch start; the inverse walks
 // the same way from the left with the slot map inverted in LDS.  Needs
 // l + 1 <= 256 * U.
-template <typename T>
+template <typename T, uint32_t NT = 256>
 __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t l, uint32_t inv,
                                          uint32_t fuse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -301,7 +301,7 @@ __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t
     const bool classic = fuse == 2;
     const uint32_t L1 = l + 1;
     constexpr int U = 8;
-    const uint32_t rows_per = (256u * U) / L1;
+    const uint32_t rows_per = (NT * U) / L1;
     // slot[x]: where rank x of a leaf block goes (its inverse when inv)
     for (uint32_t x = threadIdx.x; x < l; x += blockDim.x) {
         const uint32_t q = mveb_pos32(x, l, classic);
@@ -319,13 +319,13 @@ __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t
             const uint64_t b = (uint64_t)r * L1;
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                const uint32_t y = threadIdx.x + (uint32_t)k * NT;
                 if (y < l) v[k] = a[b + y];
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                const uint32_t y = threadIdx.x + (uint32_t)k * NT;
                 if (y < l) a[b + slot[y]] = v[k];
             }
         }
@@ -335,13 +335,13 @@ __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t
             T v[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                const uint32_t y = threadIdx.x + (uint32_t)k * NT;
                 if (y < n_el) v[k] = a[r + (uint64_t)lo * l + y];
             }
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+                const uint32_t y = threadIdx.x + (uint32_t)k * NT;
                 if (y < n_el) {
                     const uint32_t rr = y / l, q = y - rr * l, row = lo + rr;
                     a[(uint64_t)row * L1 + slot[q]] = v[k];
@@ -359,13 +359,13 @@ __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t
         const uint64_t b = (uint64_t)r * L1;
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+            const uint32_t y = threadIdx.x + (uint32_t)k * NT;
             if (y < l) v[k] = a[b + y];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const uint32_t y = threadIdx.x + (uint32_t)k * 256u;
+            const uint32_t y = threadIdx.x + (uint32_t)k * NT;
             if (y < l) a[b + slot[y]] = v[k];
         }
     }
@@ -376,13 +376,13 @@ __device__ inline void stream_fused_body(T *__restrict__ a, uint32_t r, uint32_t
         T v[U];
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const uint32_t x = threadIdx.x + (uint32_t)k * 256u;
+            const uint32_t x = threadIdx.x + (uint32_t)k * NT;
             if (x < n_el) v[k] = a[(uint64_t)lo * L1 + x];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < U; k++) {
-            const uint32_t x = threadIdx.x + (uint32_t)k * 256u;
+            const uint32_t x = threadIdx.x + (uint32_t)k * NT;
             if (x < n_el) {
                 const uint32_t rr = x / L1, cc = x - rr * L1, row = lo + rr;
                 if (cc == l)
@@ -666,6 +666,20 @@ __global__ __launch_bounds__(256) void k_multi_stream(const MStream *__restrict_
         stream_body<T>(a, m.r, m.l, m.inv);
 }
 
+// fused streamed gathers of many arrays, NT threads per partition: a
+// partition's workgroup finishes NT / 256 times sooner than with 256 threads,
+// so a launch whose partitions fill the device a little over a whole number
+// of times loses less to its last, partial round
+template <typename T, uint32_t NT>
+__global__ __launch_bounds__(NT) void k_multi_stream_fused(const MStream *__restrict__ d,
+                                                           const uint64_t *__restrict__ pre,
+                                                           uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MStream m = d[i];
+    T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, blockIdx.x - pre[i]);
+    stream_fused_body<T, NT>(a, m.r, m.l, m.inv, m.fuse);
+}
+
 // moves of many chained ops, numbered op after op: a move waits only for
 // lower-numbered moves of its own op, which hold lower tickets
 template <typename T, int PER, int MODE>
@@ -728,9 +742,19 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
                            (const MLds *)m.desc, pre, m.nd);
         break;
     case MK_STREAM:
-    case MK_FUSED:
         hipLaunchKernelGGL(k_multi_stream<T>, dim3((uint32_t)m.tasks), dim3(256), m.lds, s,
                            (const MStream *)m.desc, pre, m.nd);
+        break;
+    case MK_FUSED:
+        if (m.threads == 1024)
+            hipLaunchKernelGGL((k_multi_stream_fused<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024),
+                               m.lds, s, (const MStream *)m.desc, pre, m.nd);
+        else if (m.threads == 512)
+            hipLaunchKernelGGL((k_multi_stream_fused<T, 512>), dim3((uint32_t)m.tasks), dim3(512),
+                               m.lds, s, (const MStream *)m.desc, pre, m.nd);
+        else
+            hipLaunchKernelGGL((k_multi_stream_fused<T, 256>), dim3((uint32_t)m.tasks), dim3(256),
+                               m.lds, s, (const MStream *)m.desc, pre, m.nd);
         break;
     case MK_CHAIN: {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(m.tasks, (uint64_t)c->num_cus * 8);
