@@ -914,7 +914,7 @@ static tl_status build_planned(Batch &B, Ctx *c) {
 // to a different array, so the launches of a wave are independent; a
 // rotation's stash copy runs with the wave's copies and its lane walk after
 // them.  Reversals and unit merges (rare in batches) run as single ops.
-// threads per workgroup of the batch's fused streamed gathers:
+// threads per workgroup of the batch's streamed gathers:
 // $TL_STREAM_THREADS (256, 512 or 1024), else 1024
 static uint32_t stream_threads() {
     const char *e = getenv("TL_STREAM_THREADS");
@@ -1029,8 +1029,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 const MStream d{A, runs, (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
                                 (uint32_t)op.p[2], (uint32_t)op.p[3], 0};
                 const bool fused = op.p[3] != 0;
-                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, fused ? stream_threads() : 256, &d,
-                    sizeof d, op.nitems, fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
+                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, stream_threads(), &d, sizeof d,
+                    op.nitems, fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
                 break;
             }
             case OP_CHAINED: {

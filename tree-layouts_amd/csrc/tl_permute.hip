@@ -653,15 +653,17 @@ __global__ __launch_bounds__(512) void k_multi_lds(const MLds *__restrict__ d,
                 m.rows, m.magic, m.inv, blockIdx.x - pre[i]);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_multi_stream(const MStream *__restrict__ d,
-                                                      const uint64_t *__restrict__ pre,
-                                                      uint32_t nd) {
+// NT threads per partition (the stream's batches are NT * 8 keys; any batch
+// size keeps every batch read before it is written)
+template <typename T, uint32_t NT>
+__global__ __launch_bounds__(NT) void k_multi_stream(const MStream *__restrict__ d,
+                                                     const uint64_t *__restrict__ pre,
+                                                     uint32_t nd) {
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MStream m = d[i];
     T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, blockIdx.x - pre[i]);
     if (m.fuse)
-        stream_fused_body<T>(a, m.r, m.l, m.inv, m.fuse);
+        stream_fused_body<T, NT>(a, m.r, m.l, m.inv, m.fuse);
     else
         stream_body<T>(a, m.r, m.l, m.inv);
 }
@@ -742,8 +744,15 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
                            (const MLds *)m.desc, pre, m.nd);
         break;
     case MK_STREAM:
-        hipLaunchKernelGGL(k_multi_stream<T>, dim3((uint32_t)m.tasks), dim3(256), m.lds, s,
-                           (const MStream *)m.desc, pre, m.nd);
+        if (m.threads == 1024)
+            hipLaunchKernelGGL((k_multi_stream<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024),
+                               m.lds, s, (const MStream *)m.desc, pre, m.nd);
+        else if (m.threads == 512)
+            hipLaunchKernelGGL((k_multi_stream<T, 512>), dim3((uint32_t)m.tasks), dim3(512), m.lds,
+                               s, (const MStream *)m.desc, pre, m.nd);
+        else
+            hipLaunchKernelGGL((k_multi_stream<T, 256>), dim3((uint32_t)m.tasks), dim3(256), m.lds,
+                               s, (const MStream *)m.desc, pre, m.nd);
         break;
     case MK_FUSED:
         if (m.threads == 1024)
