@@ -62,6 +62,21 @@ def test_permute_stash_budgets(layout, b, row_stash, dev, monkeypatch):
         assert np.array_equal(got, exp), (layout, b, row_stash, n)
 
 
+@pytest.mark.parametrize("tile_k", ["5", "6"])
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+def test_bst_bitrev_tile_sizes(tile_k, kind, dev, monkeypatch):
+    """32x32 (the default) and 64x64 bit-reversal tiles give the same BST
+    layout, and the inverse restores sorted order"""
+    monkeypatch.setenv("TL_BITREV_K", tile_k)
+    for n in [4095, 16383, 16385, 65535, 300007, 1048575, 1048576, 3_000_017]:
+        got, exp = _run("bst", 1, kind, n, dev)
+        assert np.array_equal(got, exp), (tile_k, kind, n)
+        A = torch.from_numpy(got.astype(np.int32 if kind == "u32" else np.int64)).to(dev)
+        tl_amd.unpermute("bst", A, kind=kind)
+        torch.cuda.synchronize()
+        assert np.array_equal(A.cpu().numpy().astype(np.int64), np.arange(n)), (tile_k, kind, n)
+
+
 @pytest.mark.parametrize("layout,b,n", [("bst", 1, (1 << 24) - 1), ("bst", 1, (1 << 24) + 12345),
                                         ("btree", 16, 17 ** 6 - 1), ("btree", 16, 30_000_001),
                                         ("btree", 8, 9 ** 7 + 5), ("mveb", 0, (1 << 24) - 1),

@@ -160,7 +160,7 @@ struct Tuning {
     uint64_t row_stash_max = 1ull << 24;   // separators a row slide may stash
     int64_t row_stash_env = -1;            // $TL_ROW_STASH_MAX when set (0: no row slides)
     uint64_t pack_bytes = 32768;           // keys of several small in-LDS items per workgroup
-    uint32_t bitrev_k = 0;                 // bit-reversal tile 2^K x 2^K (0: 6 for 4-byte keys, 5 for 8-byte)
+    uint32_t bitrev_k = 0;                 // bit-reversal tile 2^K x 2^K (0: 5, both key widths)
     bool rotate_walk = true;               // one-pass rotations (else three reversals)
     bool fuse_leaves = true;               // vEB leaf layouts fused into streamed gathers
     uint64_t walk_lanes = 1ull << 24;      // lanes (stashed keys) of a one-pass rotation
@@ -536,7 +536,7 @@ inline void plan_bst(Planner &P, uint64_t n) {
     } else {
         d = h + 1;
     }
-    const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : ((P.W == 4) ? 6 : 5);
+    const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : 5u;
     if (n <= P.cap() || d < 2 * K + 2) {
         P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
         return;
