@@ -1146,25 +1146,31 @@ static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
     TL_TRY(fill_u32_launch(c->err_flag, 1, 0u, s));
     TL_TRY(fill_u32_launch(B.d_pflags, B.flag_stride / 4, 0u, s));
     if (B.waves) {
-        // permutations wave by wave, every launch over many arrays
+        // permutations wave by wave, every launch over many arrays, on one
+        // branch; the whole-array in-LDS launches of the small arrays, which
+        // touch no other array, on the other branches beside them
         TL_TRY(fill_u32_launch(B.d_wctr, B.nctr, 0u, s));
+        TL_TRY(fork_branches(B, 0, s));
         for (size_t g = 0; g < B.small.size(); g++) {
             const SmallGroup &G = B.small[g];
             TL_TRY(lds_items_launch((const uint64_t *)(base + G.at), (uint32_t)(G.desc.size() / 2),
-                                    G.W, G.kind, G.b, G.max_n, s));
+                                    G.W, G.kind, G.b, G.max_n,
+                                    B.sub[1 + g % (Batch::NSUB - 1)]));
         }
+        hipStream_t ws = B.sub[0];
         for (const WaveStep &st : B.steps) {
             if (st.multi) {
-                TL_TRY(multi_launch(c, st.m, s));
+                TL_TRY(multi_launch(c, st.m, ws));
                 continue;
             }
             const LargeItem &L = B.large[st.large];
             Plan one;
             one.ops.push_back(L.plan.ops[st.op]);
             one.max_stash = L.plan.max_stash;
-            TL_TRY(plan_launch(c, one, B.items[L.item].A, L.W, s, (const uint64_t *)(base + L.at),
+            TL_TRY(plan_launch(c, one, B.items[L.item].A, L.W, ws, (const uint64_t *)(base + L.at),
                                (unsigned char *)B.d_wstash + B.wstash_at[st.large], nullptr));
         }
+        TL_TRY(join_branches(B, 0, s));
         return run_queries(B, base, s);
     }
     // permutations: every array on one branch, with that branch's stash and flags
