@@ -158,6 +158,25 @@ def test_btree_query_duplicate_keys(wide, b, dev, monkeypatch):
         assert np.array_equal(got, ol.search("btree", A, Q, "u64", b)), (b, n)
 
 
+@pytest.mark.parametrize("top", ["0", "1"])
+@pytest.mark.parametrize("qpl", ["1", "2", "4"])
+@pytest.mark.parametrize("layout", ["mveb", "veb"])
+@pytest.mark.parametrize("kind", ["u32", "i64"])
+def test_veb_query_shapes(top, qpl, layout, kind, dev, monkeypatch):
+    """The vEB query with and without its BFS-staged top levels and with 1,
+    2 or 4 queries per lane gives the oracle's answers (perfect and
+    non-perfect sizes, every table-chain shape in the list)"""
+    monkeypatch.setenv("TL_VEB_TOP", top)
+    monkeypatch.setenv("TL_VEB_QPL", qpl)
+    rng = np.random.default_rng(21)
+    for n in [1, 2, 3, 7, 100, 1023, 1024, 1025, 4095, 65535, 100003, 1048575, 1000003]:
+        keys = make_keys(kind, n, rng)
+        A = ol.permute(layout, keys.copy(), 0)
+        Q = make_queries(kind, keys, 8192, rng)
+        got = gpu_answers(layout, 0, kind, A, Q, dev)
+        assert np.array_equal(got, ol.search(layout, A, Q, kind, 0)), (layout, kind, n)
+
+
 @pytest.mark.parametrize("kind", ["u32", "u64", "i32", "i64"])
 def test_binary_search_matches_oracle(kind, dev):
     """Sorted array: hit -> index, miss -> predecessor's index, below the
