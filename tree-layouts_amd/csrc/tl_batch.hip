@@ -909,11 +909,6 @@ static tl_status build_planned(Batch &B, Ctx *c) {
     return TL_OK;
 }
 
-// Groups the k-th op of every large array by kernel (kind, key width and
-// template parameter) into one launch each.  Within a wave every op belongs
-// to a different array, so the launches of a wave are independent; a
-// rotation's stash copy runs with the wave's copies and its lane walk after
-// them.  Reversals and unit merges (rare in batches) run as single ops.
 // threads per workgroup of the batch's streamed gathers:
 // $TL_STREAM_THREADS (256, 512 or 1024), else 1024
 static uint32_t stream_threads() {
@@ -922,6 +917,12 @@ static uint32_t stream_threads() {
     return (v == 256 || v == 512) ? (uint32_t)v : 1024u;
 }
 
+// The large arrays' plans as a sequence of multi-array launches.  A launch
+// groups ops of one kernel (kind, key width, template parameter, threads),
+// at most one op per array, so its ops are independent; every array's ops
+// keep their plan order (the schedule is below).  A rotation's stash copy
+// and its lane walk are two steps.  Reversals and unit merges (rare in
+// batches) run as single ops.
 static tl_status build_waves(Batch &B, Ctx *c) {
     const unsigned char *pbase = (const unsigned char *)B.d_pdata;
     // a stash of its own for every array: their ops run together
