@@ -14,18 +14,20 @@
 namespace tl {
 
 // ---------------------------------------------------------------------------
-// BST: one lane per query.  The top BST_TOP keys (the first levels) are staged
-// in LDS; each lane keeps BST_QPL queries in flight so that independent
-// dependent descents overlap their memory latency.
-constexpr uint32_t BST_TOP = 2047;
+// BST: one lane per query.  The top levels are staged in LDS, 16 KiB of keys
+// (11 levels of 8-byte keys, 12 of 4-byte keys); each lane keeps BST_QPL
+// queries in flight so that independent dependent descents overlap their
+// memory latency.
+template <typename K>
+constexpr uint32_t BST_TOP = sizeof(K) == 4 ? 4095 : 2047;
 constexpr int BST_QPL = 4;   // queries in flight per lane
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint64_t n,
                                                    const K *__restrict__ Q, uint64_t nq,
                                                    uint64_t *__restrict__ ans) {
-    __shared__ K top[BST_TOP];
-    const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP, n);
+    __shared__ K top[BST_TOP<K>];
+    const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP<K>, n);
     for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
     __syncthreads();
     // BST_QPL independent queries per lane, strided by the grid for coalescing
@@ -178,15 +180,17 @@ __global__ __launch_bounds__(256) void k_query_btree_lane(const K *__restrict__ 
 // keeps the upper or lower part.  The len sequence does not depend on q, so
 // every query takes the same ceil(log2 n) steps and a wave never diverges.
 // The probes of the first BIN_TOPL steps form a complete binary tree, staged
-// per workgroup in LDS in heap order; the answer is base when A[base] <= q.
-constexpr uint32_t BIN_TOPL = 11;   // 2047 probe keys in LDS
+// per workgroup in LDS in heap order (16 KiB: 11 steps of 8-byte keys, 12 of
+// 4-byte keys); the answer is base when A[base] <= q.
+template <typename K>
+constexpr uint32_t BIN_TOPL = sizeof(K) == 4 ? 12 : 11;
 constexpr int BIN_QPL = 4;          // queries in flight per lane
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_query_binary(const K *__restrict__ A, uint64_t n,
                                                       const K *__restrict__ Q, uint64_t nq,
                                                       uint64_t *__restrict__ ans) {
-    __shared__ K top[1u << BIN_TOPL];   // top[0] = A[0], node i >= 1 in heap order
+    __shared__ K top[1u << BIN_TOPL<K>];   // top[0] = A[0], node i >= 1 in heap order
     __shared__ uint64_t half[64];
     __shared__ uint32_t steps_s;
     if (threadIdx.x == 0) {
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(256) void k_query_binary(const K *__restrict__ A, u
         steps_s = t;
     }
     __syncthreads();
-    const uint32_t S = steps_s, T = min(S, BIN_TOPL);
+    const uint32_t S = steps_s, T = min(S, BIN_TOPL<K>);
     if (n) {
         // node i at level t probes base(i) + half[t], base(i) = the halves
         // taken on the path from the root (the bits of i below its top bit)
@@ -618,7 +622,8 @@ static void launch_mveb(Ctx *c, const void *A, uint64_t n, const tl_veb_table *t
     uint32_t topmax;
     veb_shape(qpl, topmax);
     const bool narrow = n < (1ull << 32);
-    uint32_t ntop = (uint32_t)std::min<uint64_t>(n, topmax);
+    // 16 KiB of staged keys: 2047 8-byte or 4095 4-byte keys
+    uint32_t ntop = (uint32_t)std::min<uint64_t>(n, topmax ? (sizeof(K) == 4 ? 4095u : topmax) : 0u);
     size_t bytes = veb_smem(table_size, ntop, sizeof(K), narrow ? 4 : 8, qpl).bytes;
     if (bytes > 64 * 1024) {   // a long table chain: no staged top keys
         ntop = 0;
