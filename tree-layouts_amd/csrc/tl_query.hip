@@ -622,14 +622,16 @@ static void launch_mveb(Ctx *c, const void *A, uint64_t n, const tl_veb_table *t
     uint32_t topmax;
     veb_shape(qpl, topmax);
     const bool narrow = n < (1ull << 32);
-    // 16 KiB of staged keys: 2047 8-byte or 4095 4-byte keys
+    const uint32_t grid = grid_for(c, nq, 256 * qpl);
+    // 16 KiB of staged keys (2047 8-byte or 4095 4-byte keys), when every
+    // workgroup has at least 1024 queries to spend them on
     uint32_t ntop = (uint32_t)std::min<uint64_t>(n, topmax ? (sizeof(K) == 4 ? 4095u : topmax) : 0u);
+    if (nq < (uint64_t)grid * 1024) ntop = 0;
     size_t bytes = veb_smem(table_size, ntop, sizeof(K), narrow ? 4 : 8, qpl).bytes;
     if (bytes > 64 * 1024) {   // a long table chain: no staged top keys
         ntop = 0;
         bytes = veb_smem(table_size, ntop, sizeof(K), narrow ? 4 : 8, qpl).bytes;
     }
-    const uint32_t grid = grid_for(c, nq, 256 * qpl);
 #define TL_V(QQ)                                                                                   \
     if (qpl == QQ) {                                                                               \
         if (narrow)                                                                                \
