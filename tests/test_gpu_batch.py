@@ -114,6 +114,28 @@ def test_batch_planned_path_matches_oracle(lds, waves, dev, monkeypatch):
     B.close()
 
 
+@pytest.mark.parametrize("knobs", [{"TL_ROW_STASH_MAX": "0"}, {"TL_WALK_LANES": "0"},
+                                   {"TL_ROW_STASH_MAX": "0", "TL_WALK_LANES": "0"}])
+def test_batch_schedule_with_single_ops(knobs, dev, monkeypatch):
+    """Plans with unit merges (no row slides) and rotations by reversals (no
+    lane walks): ops without a multi-array kernel run alone inside the
+    multi-array launch schedule, which must keep every array's ops in order"""
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    monkeypatch.setenv("TL_BATCH_WAVES", "1")
+    arrays, host = make_batch(120, 6, 18, 11, dev)
+    B = tl_amd.Batch(arrays)
+    B.run(graph=False)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    restore(arrays, host, dev)
+    torch.cuda.synchronize()
+    B.run(graph=True)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    B.close()
+
+
 def test_batch_planned_path_autotune(dev):
     arrays, host = make_batch(90, 6, 16, 7, dev)
     cfg = tl_amd.batch_autotune(arrays, sample=30)
