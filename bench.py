@@ -420,8 +420,10 @@ def main():
                     help="LDS budget of the batch's plans in bytes (0: the library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the layouts would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(a permute would split the ranks into one contiguous slice per "
+              "GPU and exchange keys by destination with one all-to-all)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

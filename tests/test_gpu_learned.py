@@ -1,6 +1,6 @@
 """Learned start node: answers equal the plain root descent bit for bit (the
-plain descent is the fallback and the judge), the model's error bound holds
-on every key, and hits never fall back."""
+plain descent is the fallback and decides every answer), the model's error
+bound holds on every key, and hits never fall back."""
 import numpy as np
 import pytest
 import torch
