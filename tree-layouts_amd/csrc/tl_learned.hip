@@ -617,8 +617,10 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
         wide_group(b, sizeof(K)) >= 2) {
         const uint32_t G = wide_group(b, sizeof(K));
         const uint64_t want = (nq + 256 - 1) / 256;
+        // many more workgroups than stay resident (256 per CU) run faster
+        // for this kernel, which stages no keys (8.6 against 9.5 ms with 16)
         const uint32_t grid = (uint32_t)std::max<uint64_t>(
-            1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 16));
+            1, std::min<uint64_t>(want, (uint64_t)c->num_cus * 256));
         switch (G) {
 #define TL_F(GG)                                                                                   \
     case GG:                                                                                       \
