@@ -50,6 +50,64 @@ def gen_keys(n, seed, dev):
     return keys, g
 
 
+def timing_stats(count, runs_ms, scale=1e-9):
+    """printTimings-style summary of per-run times (cpu/src/common.cpp:31-58,
+    gpu/src/common.cu:41-67): mean and sd of the times, the throughput of the
+    mean time (count per second * scale) and the spread of the per-run
+    throughputs about it, and every run."""
+    t = np.asarray(runs_ms, dtype=np.float64)
+    mean = float(t.mean())
+    tp = count / (t * 1e-3) * scale
+    tp_mean = count / (mean * 1e-3) * scale
+    return {"runs": len(t), "mean_ms": round(mean, 4), "sd_ms": round(float(t.std()), 4),
+            "tp_mean": round(tp_mean, 3),
+            "tp_sd": round(float(np.sqrt(np.mean((tp - tp_mean) ** 2))), 3),
+            "runs_ms": [round(float(x), 4) for x in t]}
+
+
+def event_steps(fn, steps, stream):
+    """Runs fn() `steps` times on `stream` with an event between calls;
+    returns (total ms, per-step ms)."""
+    import torch
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record(stream)
+    for i in range(steps):
+        fn()
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+    return evs[0].elapsed_time(evs[-1]), per
+
+
+def btree_query_bytes(n, b, w, cache_bytes=256 << 20):
+    """Algorithmic bytes per uniform-hit query of a level-order B-tree: the
+    query key, the 8-byte answer and one b*w node per visited level.  Returns
+    (all levels, DRAM-resident levels only): levels are counted from the root
+    while their cumulative size fits the on-die cache (the 256 MiB Infinity
+    Cache), and a hit visits a level with the probability that its key lies
+    at that level or below."""
+    k1 = b + 1
+    h, p = 0, 1                            # p = k1^h <= n < k1^(h+1)
+    while p * k1 <= n:
+        p *= k1
+        h += 1
+    if n == p * k1 - 1:                    # perfect: h + 1 full levels
+        levels = [k1 ** t * b for t in range(h + 1)]
+    else:                                  # perfect internal tree + leaf level
+        levels = [k1 ** t * b for t in range(h)] + [n - (p - 1)]
+    below = n
+    all_b, dram_b, cum = 0.0, 0.0, 0
+    for keys_here in levels:
+        reach = below / n                  # P(a hit reaches this level)
+        cum += keys_here * w
+        all_b += reach * b * w
+        if cum > cache_bytes:
+            dram_b += reach * b * w
+        below -= keys_here
+    return w + 8 + all_b, w + 8 + dram_b
+
+
 def sample_check_layout(A, keys_sorted, n, b, rng, samples=200_000):
     """slot of sampled ranks from the oracle's closed form; checker only"""
     import ctypes
@@ -77,36 +135,90 @@ def measured_traffic(name):
     return None
 
 
-def cpu_baseline(keys_np, b, nq_sample, rng):
-    """Oracle OpenMP restatement of the reference CPU path (btree-cycles +
-    query-btree), on a bounded sample, on the host cores."""
+def measured_traffic_entry(name):
+    """All 'key = value' lines of a committed rocprofv3 summary's 'Bench:'
+    block, or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    out, on = {}, False
+    for line in lines:
+        if line.startswith("Bench:"):
+            on = True
+            continue
+        if on:
+            if "=" not in line:
+                break
+            k, v = line.split("=", 1)
+            try:
+                out[k.strip()] = float(v.split()[0])
+            except ValueError:
+                out[k.strip()] = v.strip()
+    return out or None
+
+
+def cpu_baseline(keys_np, b, nq_sample, rng, iters=10, budget_s=30.0):
+    """The reference's CPU B-tree path (btree-cycles permute, query-btree
+    search; cpu/src/btree-cycles.h:71-96, cpu/src/query-btree.h:59-64), as
+    compiled from the reference sources into oracle/_ref/libref.so, timed like
+    the reference's drivers: `iters` runs, the sorted input restored before
+    each permute (outside the timed region), query generation excluded
+    (cpu/test/build-btree-cycles.cpp, cpu/src/query-btree.h:68-80).  Without
+    that library the oracle's OpenMP restatement runs instead, which is about
+    15x slower than the reference (kind "port").  Runs stop early once
+    budget_s of CPU time is spent.  Threads: the CPUs this process may use,
+    at most 16 (the host share of one GPU)."""
+    import ctypes
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
-    cores = len(os.sched_getaffinity(0))
-    cores = max(1, min(cores, 16))
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
     n = keys_np.size
-    times = []
-    for _ in range(3):
-        a = keys_np.copy()
+    r = ol.ref()
+    kind = "reference" if r is not None else "port"
+    a = np.empty_like(keys_np)
+    times, spent = [], 0.0
+    for _ in range(iters):
+        np.copyto(a, keys_np)
         t0 = time.perf_counter()
-        ol.permute("btree", a, b, p=cores)
-        times.append(time.perf_counter() - t0)
+        if r is not None:
+            r.ref_permute_btree_u64(a.ctypes.data_as(ol.VP), n, b, cores)
+        else:
+            ol.permute("btree", a, b, p=cores)
+        dt = time.perf_counter() - t0
+        times.append(dt * 1e3)
+        spent += dt
+        if spent > budget_s:
+            break
     q = keys_np[rng.integers(0, n, size=nq_sample)]
-    t0 = time.perf_counter()
-    ans = ol.search("btree", a, q, "u64", b, p=cores)
-    tq = time.perf_counter() - t0
-    assert (ans < n).all()
-    return {"permute_gkeys_s": n / min(times) / 1e9, "query_mq_s": nq_sample / tq / 1e6,
-            "cores": cores, "n": n, "nq": nq_sample}
+    ans = np.empty(nq_sample, dtype=np.uint64)
+    qtimes = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        if r is not None:
+            r.ref_search_btree_u64(a.ctypes.data_as(ol.VP), n, b, q.ctypes.data_as(ol.VP),
+                                   nq_sample, ans.ctypes.data_as(ol.VP), cores)
+        else:
+            ans = ol.search("btree", a, q, "u64", b, p=cores)
+        dt = time.perf_counter() - t0
+        qtimes.append(dt * 1e3)
+        spent += dt
+        if spent > 2 * budget_s:
+            break
+    assert (ans < n).all() and np.array_equal(a[ans.astype(np.int64)], q)
+    return {"kind": kind, "cores": cores, "n": n, "nq": nq_sample,
+            "permute": timing_stats(n, times), "query": timing_stats(nq_sample, qtimes, 1e-6)}
 
 
-def run_batch_permutes(args, dev):
-    """The uneven batch's permutations alone (no queries), with the arrays of
-    the batch workload (same sizes, dtypes and layouts) laid out as slices of
-    one int32 and one int64 buffer, restored by one copy each between
-    replays, so that a profiled run makes few dispatches besides the batch's
-    own.  Starts with four calibration copies of 2^27 int64 keys (1 GiB read,
-    1 GiB written) that scale the FETCH_SIZE / WRITE_SIZE counters."""
+def measure_batch_permutes(args, dev, calibrate=False):
+    """The uneven batch's permutations alone (no queries): the arrays of the
+    batch workload (4096 arrays, sizes floor(2^U), U ~ U[8, 22], seed 5,
+    int32 / uint64 50/50, BST / B-tree b=8 / mvEB round robin) as slices of
+    one int32 and one int64 buffer, restored by one copy each between graph
+    replays.  With `calibrate`, four copies of 2^27 int64 keys (1 GiB read,
+    1 GiB written) open the run to scale the FETCH_SIZE / WRITE_SIZE counters
+    of a profiled run."""
     import torch
     import tl_amd
     rng = np.random.default_rng(5)
@@ -117,11 +229,12 @@ def run_batch_permutes(args, dev):
     pad = lambda x: (x + 63) // 64 * 64
     n32 = sum(pad(int(z)) for z, k in zip(sizes, kinds) if k == "i32")
     n64 = sum(pad(int(z)) for z, k in zip(sizes, kinds) if k == "u64")
-    cal_a = torch.arange(1 << 27, device=dev, dtype=torch.int64)
-    cal_b = torch.empty_like(cal_a)
-    for _ in range(4):
-        cal_b.copy_(cal_a)
-    del cal_a, cal_b
+    if calibrate:
+        cal_a = torch.arange(1 << 27, device=dev, dtype=torch.int64)
+        cal_b = torch.empty_like(cal_a)
+        for _ in range(4):
+            cal_b.copy_(cal_a)
+        del cal_a, cal_b
     b32 = torch.arange(n32, device=dev, dtype=torch.int32)
     b64 = torch.arange(n64, device=dev, dtype=torch.int64)
     p32, p64 = b32.clone(), b64.clone()
@@ -138,8 +251,10 @@ def run_batch_permutes(args, dev):
         perm_bytes += 2.0 * n * (4 if kind == "i32" else 8)
         arrays.append({"A": A, "kind": kind, "layout": layouts[i % 3], "b": 8, "Q": None,
                        "ans": None})
+    t0 = time.perf_counter()
     P = tl_amd.Batch(arrays, lds_bytes=args.batch_lds)
-    log(f"[bench] batch permutes: {count} arrays, {keys_total} keys")
+    create_s = time.perf_counter() - t0
+    log(f"[bench] batch permutes: {count} arrays, {keys_total} keys, created in {create_s:.1f} s")
 
     def restore():
         b32.copy_(p32)
@@ -154,15 +269,24 @@ def run_batch_permutes(args, dev):
             times.append(t)
         log(f"[bench] batch permute replay {step}: {t:.3f} ms")
     P.close()
+    del P, arrays, b32, b64, p32, p64
+    torch.cuda.empty_cache()
     ms = float(np.mean(times))
     traffic = measured_traffic("batch_permutes_traffic.txt")
-    out = {"workload": "uneven batch, permutations only", "ms": round(ms, 3),
-           "Gkeys_s": round(keys_total / (ms * 1e-3) / 1e9, 3),
-           "algorithmic_frac": round(perm_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "traffic": None if traffic is None else round(traffic),
-           "traffic_frac": None if traffic is None
-           else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    print(json.dumps(out), flush=True)
+    return {"workload": f"uneven batch, permutations only: {count} arrays, {keys_total} keys, "
+                        "one HIP graph replay per step",
+            "ms": round(ms, 3), "Gkeys_s": round(keys_total / (ms * 1e-3) / 1e9, 3),
+            "timings": timing_stats(keys_total, times),
+            "algorithmic_bytes": round(perm_bytes),
+            "algorithmic_frac": round(perm_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_frac": None if traffic is None
+            else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "create_s": round(create_s, 2)}
+
+
+def run_batch_permutes(args, dev):
+    print(json.dumps(measure_batch_permutes(args, dev, calibrate=True)), flush=True)
 
 
 def run_batch(args, dev):
@@ -410,7 +534,14 @@ def main():
     ap.add_argument("--b", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--log2leaves", type=int, default=20, help="learned model leaves (log2)")
+    ap.add_argument("--log2leaves", type=int, default=16,
+                    help="learned model leaves (log2), the headline model")
+    ap.add_argument("--log2leaves-big", type=int, default=20,
+                    help="a second, larger learned model reported beside it (0: none)")
+    ap.add_argument("--skip-batch", action="store_true",
+                    help="leave the batch permutations out of the headline line")
+    ap.add_argument("--cpu-log2n", type=int, default=28, help="CPU baseline sample (log2 keys)")
+    ap.add_argument("--cpu-iters", type=int, default=10, help="CPU baseline runs")
     ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb"],
                     default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
@@ -455,17 +586,15 @@ def main():
     # ---- timed permutes (same buffer, same data movement every step)
     for _ in range(args.warmup):
         tl_amd.permute_btree(A, b, kind="u64", timed=False)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_wall = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        tl_amd.permute_btree(A, b, kind="u64", timed=False)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    total_ms, per_ms = event_steps(lambda: tl_amd.permute_btree(A, b, kind="u64", timed=False),
+                                   args.steps, stream)
     wall_ms = (time.perf_counter() - t_wall) * 1e3
-    perm_ms = ev0.elapsed_time(ev1) / args.steps
-    log(f"[bench] permute {perm_ms:.3f} ms/step (host wall {wall_ms / args.steps:.3f})")
+    tl_amd.sync()
+    perm_ms = total_ms / args.steps
+    perm_stats = timing_stats(n, per_ms)
+    log(f"[bench] permute {perm_ms:.3f} ms/step (sd {perm_stats['sd_ms']:.3f}, host wall "
+        f"{wall_ms / args.steps:.3f})")
     unperm_ms = time_unpermute(args, A, "btree", b, stream)
     log(f"[bench] unpermute {unperm_ms:.3f} ms/step")
 
@@ -475,13 +604,9 @@ def main():
     ans = torch.empty(nq, dtype=torch.int64, device=dev)
     for _ in range(max(1, args.warmup)):
         tl_amd.query_btree(A, b, Q, ans, kind="u64", timed=False)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        tl_amd.query_btree(A, b, Q, ans, kind="u64", timed=False)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    q_ms = ev0.elapsed_time(ev1) / args.steps
+    q_total, q_per = event_steps(lambda: tl_amd.query_btree(A, b, Q, ans, kind="u64", timed=False),
+                                 args.steps, stream)
+    q_ms = q_total / args.steps
     plain_ok = bool(((ans >= 0) & (ans < n)).all().item()) and bool(
         torch.equal(A[ans.clamp(0, n - 1)], Q))
     log(f"[bench] plain query {q_ms:.3f} ms ({nq / q_ms / 1e3:.1f} Mq/s) ok={plain_ok}")
@@ -491,29 +616,52 @@ def main():
     bans = torch.empty_like(ans)
     for _ in range(max(1, args.warmup)):
         tl_amd.query_binary(keys, Q, bans, kind="u64", timed=False)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        tl_amd.query_binary(keys, Q, bans, kind="u64", timed=False)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    bs_ms = ev0.elapsed_time(ev1) / args.steps
+    bs_total, _ = event_steps(lambda: tl_amd.query_binary(keys, Q, bans, kind="u64", timed=False),
+                              args.steps, stream)
+    bs_ms = bs_total / args.steps
     bs_ok = bool(((bans >= 0) & (bans < n)).all().item()) and bool(
         torch.equal(keys[bans.clamp(0, n - 1)], Q))
     del bans
     log(f"[bench] binary search {bs_ms:.3f} ms ({nq / bs_ms / 1e3:.1f} Mq/s) ok={bs_ok}")
 
-    learned = None
-    if not args.skip_learned and hasattr(tl_amd, "learned_fit"):
+    # ---- learned start node: the configuration's model (2^16 leaves) is the
+    # headline; a larger leaf count is reported beside it
+    learned, learned_big = None, None
+    if not args.skip_learned:
         learned = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n,
                                        num_leaves=1 << args.log2leaves)
+        learned["vs_plain"] = round(q_ms / learned["ms"], 4)
+        log(f"[bench] learned (2^{args.log2leaves} leaves) {learned}")
+        if args.log2leaves_big and args.log2leaves_big != args.log2leaves:
+            learned_big = tl_amd.bench_learned(A, keys, Q, ans, b, args.steps, args.warmup, n,
+                                               num_leaves=1 << args.log2leaves_big)
+            learned_big["vs_plain"] = round(q_ms / learned_big["ms"], 4)
+            log(f"[bench] learned (2^{args.log2leaves_big} leaves) {learned_big}")
+
+    # query roofline: algorithmic bytes of the DRAM-resident levels
+    q_all_b, q_dram_b = btree_query_bytes(n, b, 8)
+
+    def qroof(ms):
+        ach = nq * q_dram_b / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "bytes_per_query": round(q_dram_b, 1),
+                "bytes_per_query_all_levels": round(q_all_b, 1), "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+    q_traffic = measured_traffic_entry("btree_b16_n2p30_u64_query_traffic.txt")
+
+    # ---- the uneven batch's permutations (4096 arrays, one graph replay a step)
+    batch = None
+    if not args.skip_batch:
+        del A, Q, ans
+        torch.cuda.empty_cache()
+        batch = measure_batch_permutes(args, dev)
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline sample")
-        ns = 1 << 25
-        keys_np = keys[:ns].cpu().numpy()
-        cpu = cpu_baseline(keys_np, b, 1 << 22, rng)
+        ns = 1 << min(args.cpu_log2n, args.log2n)
+        log(f"[bench] CPU baseline: first 2^{ns.bit_length() - 1} keys, {args.cpu_iters} runs")
+        keys_np = keys[:ns].cpu().numpy().view(np.uint64)
+        cpu = cpu_baseline(keys_np, b, 1 << 22, rng, iters=args.cpu_iters)
         log(f"[bench] CPU {cpu}")
 
     algo_bytes = 2.0 * n * 8
@@ -521,6 +669,18 @@ def main():
     traffic = None
     if n == 1 << 30 and b == 16:
         traffic = measured_traffic("btree_b16_n2p30_u64_permute_traffic.txt")
+    queries = {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
+               "plain_timings": timing_stats(nq, q_per, 1e-6),
+               "plain_roofline": qroof(q_ms),
+               "learned": learned,
+               "binary_search": {"ms": round(bs_ms, 4), "Mq_s": round(nq / bs_ms / 1e3, 2),
+                                 "answers_ok": bs_ok}}
+    if learned is not None:
+        queries["learned_roofline"] = qroof(learned["ms"])
+    if learned_big is not None:
+        queries["learned_more_leaves"] = learned_big
+    if q_traffic is not None:
+        queries["measured_traffic"] = q_traffic
     out = {
         "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
         "value": round(n / (perm_ms * 1e-3) / 1e9, 3),
@@ -537,11 +697,10 @@ def main():
         "config": {"workload": "level-order B-tree b=16, in-place permute of n=2^30 uint64 keys; "
                                "2^27 uniform-hit queries (plain and learned start)",
                    "n": n, "b": b, "queries": nq},
+        "permute_timings": perm_stats,
         "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
-        "queries": {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
-                    "learned": learned,
-                    "binary_search": {"ms": round(bs_ms, 4), "Mq_s": round(nq / bs_ms / 1e3, 2),
-                                      "answers_ok": bs_ok}},
+        "queries": queries,
+        "batch_permutes": batch,
         "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -553,10 +712,13 @@ def main():
                      "kernel": "one tl_permute_btree call (all its launches); traffic from "
                                "profiles/btree_b16_n2p30_u64_permute_traffic.txt"},
         "cpu_baseline": None if cpu is None else {
-            "value": round(cpu["permute_gkeys_s"], 4), "unit": "Gkeys/s", "cores": cpu["cores"],
-            "kind": "port",
-            "sample": f"oracle OpenMP restatement of btree-cycles, n=2^25 prefix of the same "
-                      f"keys, best of 3; query {cpu['query_mq_s']:.2f} Mq/s on 2^22 hits"},
+            "value": round(cpu["permute"]["tp_mean"], 4), "unit": "Gkeys/s",
+            "cores": cpu["cores"], "kind": cpu["kind"],
+            "sample": (f"{'reference btree-cycles compiled from its sources (oracle/_ref)' if cpu['kind'] == 'reference' else 'oracle OpenMP restatement of btree-cycles (about 15x slower than the reference)'}"
+                       f", n=2^{cpu['n'].bit_length() - 1} prefix of the same keys, b={b}, "
+                       f"{cpu['permute']['runs']} runs with the sorted input restored before "
+                       f"each; query-btree on 2^22 hits: {cpu['query']['tp_mean']:.2f} Mq/s"),
+            "permute": cpu["permute"], "query": cpu["query"]},
     }
     print(json.dumps(out), flush=True)
 

@@ -8,6 +8,16 @@
  * functions are synchronous and return a time); when `ms` is NULL the call
  * only enqueues work on `stream`.
  *
+ * Permutations that use chained moves wait inside their kernels, and every
+ * such wait is bounded in time.  A wait that gives up sets a per-device error
+ * word; a timed call reports it as TL_ETIMEOUT when it returns, an untimed
+ * call leaves it for tl_sync().  The word is sticky: it holds any give-up
+ * since the last report, and reading it clears it.
+ *
+ * Scratch (stash, move flags, plan tables, error word) is per device and
+ * owned by the library.  Calls that overlap on one device -- on two streams,
+ * or from two host threads -- share it and race: serialise them.
+ *
  * Query answers are always uint64: the slot of the query key in the layout,
  * or n when the key is absent (the reference's own miss value,
  * cpu/src/query-bst.h:42, cpu/src/query-btree.h:53).
@@ -42,6 +52,18 @@ typedef struct {
 
 const char *tl_last_error(void);
 const char *tl_version(void);
+
+/* Waits for `stream`, then returns TL_ETIMEOUT (and clears the error word)
+ * if an in-kernel bounded wait gave up since the last report, else TL_OK.
+ * No counterpart in the reference, which has no error path. */
+tl_status tl_sync(void *stream);
+
+/* Diagnostic: runs one workgroup whose bounded wait can never be satisfied,
+ * bounded at wait_ms (1..60000), and returns what tl_sync returns afterwards
+ * -- TL_ETIMEOUT when the bound works.  *ms (optional) receives the elapsed
+ * GPU time.  Lets a test check the wall-clock bound on the device; no
+ * counterpart in the reference. */
+tl_status tl_selftest_timeout(uint32_t wait_ms, void *stream, float *ms);
 
 /* ---- in-place permutations of a sorted array -------------------------- */
 
@@ -161,6 +183,14 @@ uint64_t tl_learned_model_bytes(uint32_t num_leaves);
  * like tl_query_btree. */
 tl_status tl_learned_fit(const void *dev_sorted, uint64_t n, tl_dtype dtype, uint32_t num_leaves,
                          void *dev_model, void *stream, float *ms);
+
+/* Per key of dev_keys[0..n): the model's predicted rank (dev_pred) and the
+ * error bound of the leaf the key maps to (dev_err), both int64, computed
+ * with the query path's arithmetic; -1 in both for a key whose leaf is empty.
+ * For the sorted keys the model was fitted on, |pred - rank| <= err holds for
+ * every key.  Lets a caller (or a test) check the bound; asynchronous. */
+tl_status tl_learned_predict(const void *dev_model, const void *dev_keys, uint64_t n,
+                             tl_dtype dtype, int64_t *dev_pred, int64_t *dev_err, void *stream);
 
 /* tl_query_btree with the descent started at the deepest node whose subtree
  * holds the model's rank window; queries the window does not settle are

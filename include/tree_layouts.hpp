@@ -17,6 +17,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -25,6 +26,11 @@
 #include "tree_layouts.h"
 
 typedef tl_veb_table vEB_table;
+
+// timed runs per measurement in the reference's drivers (gpu/src/params.cuh:21)
+#ifndef ITERS
+#define ITERS 10
+#endif
 
 namespace tl_detail {
 template <typename T> struct dtype_of;
@@ -55,6 +61,24 @@ std::vector<T> random_hits(const T *A, uint64_t n, uint64_t q, uint64_t seed) {
     return out;
 }
 
+// Optional answer sink for tests: when set, the next timeQuery* call copies
+// its uint64 answers (slot of each query, n for a miss) into it.  The
+// reference's functions keep no answers (they overwrite a buffer they free).
+inline std::vector<uint64_t> *&answer_sink() {
+    static thread_local std::vector<uint64_t> *sink = nullptr;
+    return sink;
+}
+
+// mean and (population) standard deviation of x[0..k)
+inline void mean_sd(const double *x, int k, double &mean, double &sd) {
+    mean = 0.;
+    for (int i = 0; i < k; i++) mean += x[i];
+    mean /= k;
+    sd = 0.;
+    for (int i = 0; i < k; i++) sd += (x[i] - mean) * (x[i] - mean);
+    sd = sqrt(sd / k);
+}
+
 // uploads A and the queries, runs `search`, returns its kernel time
 template <typename T, typename F>
 float run_queries(T *A, T *dev_A, uint64_t n, uint64_t q, uint64_t seed, F search) {
@@ -70,7 +94,15 @@ float run_queries(T *A, T *dev_A, uint64_t n, uint64_t q, uint64_t seed, F searc
     if (hipMemcpy(dev_A, A, n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess &&
         hipMemcpy(dq, hq.data(), q * sizeof(T), hipMemcpyHostToDevice) == hipSuccess) {
         float t = 0.f;
-        if (search(dq, dans, &t)) ms = t;
+        if (search(dq, dans, &t)) {
+            ms = t;
+            if (std::vector<uint64_t> *sink = answer_sink()) {
+                sink->resize(q);
+                if (hipMemcpy(sink->data(), dans, q * sizeof(uint64_t), hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+                    ms = -1.f;
+            }
+        }
     }
     (void)hipFree(dq);
     (void)hipFree(dans);
@@ -164,4 +196,45 @@ float timeQueryvEB_nonperfect(T *A, T *dev_A, vEB_table *dev_tables, uint64_t *d
 
 inline void buildTable(vEB_table *table, uint64_t n, uint32_t d, uint32_t root_depth) {
     tl_detail::ok(tl_veb_build_table(table, n, d, root_depth), "buildTable");
+}
+
+// ---- driver helpers (gpu/src/common.cuh:33-40, gpu/src/common.cu:41-93) ------
+
+// the reference's sorted input: A[i] = i + 1
+template <typename T>
+void initSortedList(T *A, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++) A[i] = (T)(i + 1);
+}
+
+// "n mean_ms sd_ms mean_throughput sd_throughput t_1 .. t_ITERS", throughput
+// in keys per microsecond, as the reference's printTimings prints it
+inline void printTimings(uint64_t n, double time[ITERS]) {
+    double tp[ITERS], avg, sd, tp_avg, tp_sd;
+    for (int i = 0; i < ITERS; i++) tp[i] = n / (1000. * time[i]);
+    tl_detail::mean_sd(time, ITERS, avg, sd);
+    // the mean throughput is n over the mean time; its spread is taken about it
+    tp_avg = n / (1000. * avg);
+    tp_sd = 0.;
+    for (int i = 0; i < ITERS; i++) tp_sd += (tp[i] - tp_avg) * (tp[i] - tp_avg);
+    tp_sd = sqrt(tp_sd / ITERS);
+    printf("%llu %lf %lf %lf %lf ", (unsigned long long)n, avg, sd, tp_avg, tp_sd);
+    for (int i = 0; i < ITERS; i++) printf(" %lf", time[i]);
+    printf("\n");
+}
+
+// "n q mean_ms sd_ms mean_throughput sd_throughput t_1 .. t_ITERS", queries
+// per microsecond (printQueryTimings)
+inline void printQueryTimings(uint64_t n, uint64_t q, double time[ITERS]) {
+    double tp[ITERS], avg, sd, tp_avg, tp_sd;
+    for (int i = 0; i < ITERS; i++) tp[i] = q / (1000. * time[i]);
+    tl_detail::mean_sd(time, ITERS, avg, sd);
+    // the mean throughput is q over the mean time; its spread is taken about it
+    tp_avg = q / (1000. * avg);
+    tp_sd = 0.;
+    for (int i = 0; i < ITERS; i++) tp_sd += (tp[i] - tp_avg) * (tp[i] - tp_avg);
+    tp_sd = sqrt(tp_sd / ITERS);
+    printf("%llu %llu %lf %lf %lf %lf ", (unsigned long long)n, (unsigned long long)q, avg, sd,
+           tp_avg, tp_sd);
+    for (int i = 0; i < ITERS; i++) printf(" %lf", time[i]);
+    printf("\n");
 }

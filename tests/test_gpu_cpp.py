@@ -1,9 +1,15 @@
 """The reference-named C++ surface (include/tree_layouts.hpp) drives all three
-layouts and their queries from a reference-style driver."""
+layouts and their queries from a reference-style driver (ITERS runs,
+printTimings lines).  The driver checks the BST and B-tree layouts slot for
+slot and every query answer itself; the mvEB layouts it reports by digest,
+checked here against the oracle."""
 import os
+import re
 import subprocess
 
 import pytest
+
+import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
 
@@ -12,6 +18,16 @@ BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "hpp_drive
 
 def test_cpp_header_driver():
     assert os.path.exists(BIN), "tests/cpp/hpp_driver is not built (run __graft_entry__.build())"
-    out = subprocess.run([BIN, "1000003"], capture_output=True, text=True, timeout=120)
+    print("[test] hpp_driver 1000003", flush=True)
+    out = subprocess.run([BIN, "1000003"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().endswith("OK")
+    digests = re.findall(r"^digest (\w+) (\d+) ([0-9a-f]{16})$", out.stdout, re.M)
+    assert len(digests) == 3, out.stdout
+    for _, n, hexd in digests:
+        assert int(hexd, 16) == ol.digest(ol.layout_ranks("mveb", int(n))), (n, hexd)
+    # printTimings / printQueryTimings: n mean sd tp_mean tp_sd then ITERS times
+    lines = re.findall(r"^layout \d \w+ permute: (.*)$", out.stdout, re.M)
+    assert lines and all(len(line.split()) == 5 + 10 for line in lines), out.stdout
+    qlines = re.findall(r"^layout \d \w+ query: (.*)$", out.stdout, re.M)
+    assert qlines and all(len(line.split()) == 6 + 10 for line in qlines), out.stdout

@@ -58,21 +58,54 @@ def test_learned_equals_plain(kind, b, fused, dev, monkeypatch):
         assert int(stats.item()) <= nmiss + 0, (kind, b, n, int(stats.item()), nmiss)
 
 
-def test_learned_error_bound(dev):
-    rng = np.random.default_rng(5)
+@pytest.mark.parametrize("kind", ["u64", "i32"])
+@pytest.mark.parametrize("leaves", [1 << 8, 1 << 12, 1 << 16])
+def test_learned_error_bound(kind, leaves, dev):
+    """Every key of the fitted array lies inside its own prediction window:
+    |pred - rank| <= err of its leaf, and err <= the model's max_err."""
+    rng = np.random.default_rng(5 + leaves)
     n = 1 << 20
-    keys = _keys("u64", n, rng)
-    K = torch.from_numpy(keys).to(dev)
-    model, _ = tl_amd.learned_fit(K, 1 << 12, kind="u64")
-    err = tl_amd.learned_max_err(model)
-    assert 0 <= err < n
-    # recompute every key's prediction on the host from the model's leaves
-    raw = model.cpu().numpy()
-    hdr = raw[:8].view(np.uint8)
-    kmin = int(raw[0]); ra, rc, inv = np.frombuffer(hdr[16:28].tobytes(), dtype=np.float32)
-    leaves = raw[8:8 + 4 * (1 << 12)].reshape(-1, 4)
-    u = keys.astype(np.uint64)
-    x = ((u - np.uint64(kmin)).astype(np.float32) * np.float32(inv)).astype(np.float32)
-    leaf = np.clip(np.floor(np.float32(ra) * x + np.float32(rc)), 0, (1 << 12) - 1).astype(np.int64)
-    counts = (leaves[:, 3].view(np.uint64) >> np.uint64(32)).astype(np.int64)
-    assert counts.sum() == n
+    keys = _keys(kind, n, rng)
+    st = np.int32 if keys.dtype.itemsize == 4 else np.int64
+    K = torch.from_numpy(keys.view(st).copy()).to(dev)
+    model, _ = tl_amd.learned_fit(K, leaves, kind=kind)
+    pred, err = tl_amd.learned_predict(model, K, kind=kind)
+    torch.cuda.synchronize()
+    rank = torch.arange(n, device=dev, dtype=torch.int64)
+    assert bool((err >= 0).all()), "a fitted key maps to an empty leaf"
+    gap = (pred - rank).abs()
+    worst = int((gap - err).max().item())
+    assert worst <= 0, f"a key's rank lies {worst} outside its leaf's error window"
+    max_err = tl_amd.learned_max_err(model)
+    assert int(err.max().item()) == max_err
+    assert 0 <= max_err < n
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("b", [16, 8])
+def test_learned_equals_plain_duplicate_keys(b, fused, dev, monkeypatch):
+    """Runs of equal keys that cross node boundaries: every duplicate shares
+    one leaf and one prediction, so the learned start node still lies on the
+    plain descent's path and the answers agree bit for bit."""
+    monkeypatch.setenv("TL_LEARNED_FUSED", fused)
+    rng = np.random.default_rng(91 + b)
+    for n in (5000, 200_003, 1 << 20):
+        runs = rng.integers(1, 3 * b, size=n)          # run lengths up to 3 nodes
+        vals = np.cumsum(rng.integers(1, 40, size=n))
+        keys = np.repeat(vals, runs)[:n].astype(np.int64)
+        K = torch.from_numpy(keys.copy()).to(dev)
+        model, _ = tl_amd.learned_fit(K, 1 << 10, kind="u64")
+        A = K.clone()
+        tl_amd.permute_btree(A, b, kind="u64")
+        hits = keys[rng.integers(0, n, size=30000)]
+        miss = rng.integers(int(keys[0]) - 3, int(keys[-1]) + 3, size=10000)
+        q = np.concatenate([hits, miss]).astype(np.int64)
+        Q = torch.from_numpy(q).to(dev)
+        plain = torch.empty(q.size, dtype=torch.int64, device=dev)
+        lrn = torch.empty_like(plain)
+        tl_amd.query_btree(A, b, Q, plain, kind="u64")
+        tl_amd.query_btree_learned(A, b, model, Q, lrn, None, kind="u64")
+        torch.cuda.synchronize()
+        assert torch.equal(plain, lrn), (b, n)
+        exp = ol.search("btree", A.cpu().numpy().view(np.uint64), q.view(np.uint64), "u64", b)
+        assert np.array_equal(plain.cpu().numpy().view(np.uint64), exp), (b, n)

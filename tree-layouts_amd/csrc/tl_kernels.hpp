@@ -248,32 +248,32 @@ __device__ inline bool wait_flag(uint32_t *f, uint32_t *err, uint64_t limit) {
 __device__ inline void loads_done() { __builtin_amdgcn_s_waitcnt(0); }
 
 // waits for every move t' < t whose source overlaps [dst, dst+len)
-__device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
+__device__ inline bool wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
                                  uint32_t *flags, uint32_t *err, uint64_t limit) {
     if (m.kind == MV_SHIFT_RIGHT || m.kind == MV_SHIFT_LEFT) {
-        if (dst + len <= m.P || dst >= m.P + m.L) return;
+        if (dst + len <= m.P || dst >= m.P + m.L) return true;
         const uint64_t lo = dst > m.P ? (dst - m.P) / m.CH : 0;
         uint64_t hi = (dst + len - 1 - m.P) / m.CH;
         if (hi >= m.nchunk) hi = m.nchunk - 1;
         for (uint64_t c = lo; c <= hi; c++) {
             const uint64_t tt = (m.kind == MV_SHIFT_RIGHT) ? m.nchunk - 1 - c : c;
-            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return false;
         }
-        return;
+        return true;
     }
     if (m.kind == MV_ROW_SLIDE) {
         // destination keys [dst, dst + rows*l); sources are groups of CH rows
         const uint64_t dlen = len / (m.l + 1) * m.l;
-        if (dlen == 0) return;
+        if (dlen == 0) return true;
         const uint64_t G = m.CH * (m.l + 1);
         const uint64_t glo = (dst - m.P) / G;
         uint64_t ghi = (dst + dlen - 1 - m.P) / G;
         if (ghi >= m.nchunk) ghi = m.nchunk - 1;
         for (uint64_t g = glo; g <= ghi; g++) {
             const uint64_t tt = m.nchunk - 1 - g;
-            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return false;
         }
-        return;
+        return true;
     }
     // sparse slide: sources are the tail chunks (numbers 0..ntail-1, the
     // rightmost first) and the leaf blocks of rows r-1..0 after them
@@ -284,10 +284,10 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         const uint64_t z = (dst + len < T0 + m.L ? dst + len : T0 + m.L) - T0;
         for (uint64_t c = a / m.CH; c <= (z - 1) / m.CH; c++) {
             const uint64_t tt = ntail - 1 - c;
-            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return false;
         }
     }
-    if (dst >= T0) return;
+    if (dst >= T0) return true;
     const uint64_t L1 = m.l + 1;
     const uint64_t rel = dst - m.P;
     uint64_t i_lo = rel / L1, i_hi = (rel + len - 1) / L1;
@@ -300,20 +300,21 @@ __device__ inline void wait_deps(const MoveSpec &m, uint64_t t, uint64_t dst, ui
         const uint64_t c_lo = ov_lo / m.CH, c_hi = (ov_end - bs - 1) / m.CH;
         for (uint64_t c = c_lo; c <= c_hi; c++) {
             const uint64_t tt = ntail + (m.r - 1 - i) * m.nchunk + (m.nchunk - 1 - c);
-            if (tt < t && !wait_flag(&flags[tt], err, limit)) return;
+            if (tt < t && !wait_flag(&flags[tt], err, limit)) return false;
         }
     }
+    return true;
 }
 
 // waits of the inverse kinds: the sources of all moves tile [S0, S1) left
 // to right in move order, so the moves overlapping [dst, dst + span) form one
 // range of numbers
-__device__ inline void wait_deps_spread(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
+__device__ inline bool wait_deps_spread(const MoveSpec &m, uint64_t t, uint64_t dst, uint64_t len,
                                         uint32_t *flags, uint32_t *err, uint64_t limit) {
     const uint64_t S0 = m.P + m.r + (m.kind == MV_SPARSE_SPREAD ? m.k : 0);
     const uint64_t span = m.kind == MV_ROW_SPREAD ? len / m.l * (m.l + 1) : len;
     const uint64_t e = dst + span;   // written keys lie in [dst, e)
-    if (e <= S0) return;
+    if (e <= S0) return true;
     const uint64_t a = (dst > S0 ? dst : S0) - S0, z = e - 1 - S0;
     uint64_t ulo, uhi;
     if (m.kind == MV_ROW_SPREAD) {
@@ -332,7 +333,8 @@ __device__ inline void wait_deps_spread(const MoveSpec &m, uint64_t t, uint64_t 
         uhi = num(z);
     }
     for (uint64_t u = ulo; u <= uhi && u < t; u++)
-        if (!wait_flag(&flags[u], err, limit)) return;
+        if (!wait_flag(&flags[u], err, limit)) return false;
+    return true;
 }
 
 // loads of a row-spread move, indexed by the destination, which is rows of
@@ -427,16 +429,21 @@ __device__ inline void chained_move(T *__restrict__ A, const MoveSpec &ms, uint6
         }
         // every wave's loads complete before the chunk is published as read
         loads_done();
+        __shared__ uint32_t s_ok;
         __syncthreads();
         if (threadIdx.x == 0) {
             __hip_atomic_store(&flags[t], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool ok;
             if constexpr (MODE != CH_FORWARD)
-                wait_deps_spread(ms, t, dst, len, flags, err, limit);
+                ok = wait_deps_spread(ms, t, dst, len, flags, err, limit);
             else
-                wait_deps(ms, t, dst, len, flags, err, limit);
+                ok = wait_deps(ms, t, dst, len, flags, err, limit);
+            s_ok = ok ? 1u : 0u;
         }
         __syncthreads();
-        store_move<T, PER, MODE>(A, stash, ms, src, dst, len, v);
+        // a move whose wait gave up (the call then reports TL_ETIMEOUT) keeps
+        // its loads to itself: its destination may hold keys not yet read
+        if (s_ok) store_move<T, PER, MODE>(A, stash, ms, src, dst, len, v);
     }
 }
 

@@ -685,9 +685,62 @@ static tl_status query(Ctx *c, const void *A, uint64_t n, uint64_t b, const void
     return TL_OK;
 }
 
+// per key: the model's predicted rank and its leaf's error bound (-1 for both
+// when the key maps to an empty leaf), with the query path's own arithmetic
+template <typename K>
+__global__ __launch_bounds__(256) void k_learned_predict(const LearnedHeader *__restrict__ hp,
+                                                         const LeafModel *__restrict__ leaves,
+                                                         const K *__restrict__ keys, uint64_t n,
+                                                         int64_t *__restrict__ pred,
+                                                         int64_t *__restrict__ err) {
+    const LearnedHeader h = *hp;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t u = ukey<K>(keys[i]);
+        const LeafModel m = leaves[leaf_of(u, h)];
+        pred[i] = m.count ? predict(u, m) : -1;
+        err[i] = m.count ? (int64_t)m.err : -1;
+    }
+}
+
 }   // namespace tl
 
 using namespace tl;
+
+extern "C" tl_status tl_learned_predict(const void *dev_model, const void *dev_keys, uint64_t n,
+                                        tl_dtype dtype, int64_t *dev_pred, int64_t *dev_err,
+                                        void *stream) {
+    if (n == 0) return TL_OK;
+    if (!dev_model || !dev_keys || !dev_pred || !dev_err) {
+        set_error("tl_learned_predict: null buffer");
+        return TL_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const LearnedHeader *h = (const LearnedHeader *)dev_model;
+    const LeafModel *lv = (const LeafModel *)(h + 1);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536);
+    switch (dtype) {
+    case TL_U32:
+        hipLaunchKernelGGL(k_learned_predict<uint32_t>, dim3(grid), dim3(256), 0, s, h, lv,
+                           (const uint32_t *)dev_keys, n, dev_pred, dev_err);
+        break;
+    case TL_U64:
+        hipLaunchKernelGGL(k_learned_predict<uint64_t>, dim3(grid), dim3(256), 0, s, h, lv,
+                           (const uint64_t *)dev_keys, n, dev_pred, dev_err);
+        break;
+    case TL_I32:
+        hipLaunchKernelGGL(k_learned_predict<int32_t>, dim3(grid), dim3(256), 0, s, h, lv,
+                           (const int32_t *)dev_keys, n, dev_pred, dev_err);
+        break;
+    case TL_I64:
+        hipLaunchKernelGGL(k_learned_predict<int64_t>, dim3(grid), dim3(256), 0, s, h, lv,
+                           (const int64_t *)dev_keys, n, dev_pred, dev_err);
+        break;
+    default: set_error("unknown dtype"); return TL_EINVAL;
+    }
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
 
 extern "C" uint64_t tl_learned_model_bytes(uint32_t num_leaves) {
     return sizeof(LearnedHeader) + (uint64_t)num_leaves * sizeof(LeafModel);

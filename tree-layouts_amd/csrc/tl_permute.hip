@@ -807,6 +807,12 @@ tl_status multi_launch(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     return m.W == 4 ? multi_launch_t<uint32_t>(c, m, s) : multi_launch_t<uint64_t>(c, m, s);
 }
 
+// one workgroup waits for a flag that nothing sets, bounded by `limit`
+// wall-clock ticks: the bounded wait must give up and raise the error word
+__global__ void k_selftest_wait(uint32_t *never, uint32_t *err, uint64_t limit) {
+    if (threadIdx.x == 0) (void)wait_flag(never, err, limit);
+}
+
 static tl_status check_args(Layout lay, const void *dev_A, uint64_t n, uint64_t b, tl_dtype dt,
                             uint32_t &W) {
     if (!width_of(dt, W)) {
@@ -951,6 +957,29 @@ extern "C" tl_status tl_unpermute_btree(void *dev_A, uint64_t n, uint64_t b, tl_
 extern "C" tl_status tl_unpermute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
                                        float *ms) {
     return tl::permute(tl::L_MVEB, dev_A, n, 0, dtype, stream, ms, true);
+}
+
+extern "C" tl_status tl_selftest_timeout(uint32_t wait_ms, void *stream, float *ms) {
+    using namespace tl;
+    if (wait_ms == 0 || wait_ms > 60000) {
+        set_error("wait_ms must be in [1, 60000]");
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    hipStream_t s = (hipStream_t)stream;
+    void *never = nullptr;
+    TL_TRY(scratch(c, 3, 256, &never));
+    TL_TRY(fill_u32_launch((uint32_t *)never, 64, 0u, s));
+    Timer tm;
+    float t = 0.f;
+    TL_TRY(tm.start(s, &t));
+    hipLaunchKernelGGL(k_selftest_wait, dim3(1), dim3(64), 0, s, (uint32_t *)never, c->err_flag,
+                       (uint64_t)wait_ms * c->wall_khz);
+    TL_LAUNCH_CHECK();
+    TL_TRY(tm.stop());
+    if (ms) *ms = t;
+    return check_err_flag(c, s);
 }
 
 extern "C" tl_status tl_plan_export(uint32_t layout, uint64_t n, uint64_t b, tl_dtype dtype,

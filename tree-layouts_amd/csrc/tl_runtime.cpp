@@ -123,4 +123,13 @@ Timer::~Timer() {
 }   // namespace tl
 
 extern "C" const char *tl_last_error(void) { return tl::g_last_error.c_str(); }
+
+// Waits for `stream` and reports (and clears) a give-up of any in-kernel
+// bounded wait since the last report: the way an untimed call (ms == NULL),
+// or a captured graph of such calls, learns of TL_ETIMEOUT.
+extern "C" tl_status tl_sync(void *stream) {
+    tl::Ctx *c;
+    TL_TRY(tl::get_ctx(&c));
+    return tl::check_err_flag(c, (hipStream_t)stream);
+}
 extern "C" const char *tl_version(void) { return "tree-layouts-mi355x 0.1 (gfx950)"; }

@@ -89,6 +89,8 @@ _SIGS = {
     "tl_plan_export": [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                        ctypes.POINTER(ctypes.c_uint64)],
+    "tl_sync": [ctypes.c_void_p],
+    "tl_selftest_timeout": [ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
 }
 class BatchItem(ctypes.Structure):
     _fields_ = [("A", ctypes.c_void_p), ("n", ctypes.c_uint64), ("dtype", ctypes.c_int),
@@ -123,6 +125,8 @@ _OPTIONAL = {
                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
                          ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                          ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "tl_learned_predict": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "tl_query_btree_learned": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -164,14 +168,62 @@ def _stream(stream):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    """Device address of a contiguous ROCm tensor (None -> NULL).  The library
+    takes plain pointers and sizes, so a CPU tensor or a strided view would
+    be read as the wrong memory: both are rejected here."""
+    if t is None:
+        return ctypes.c_void_p(0)
+    if not t.is_cuda:
+        raise ValueError("tree-layouts: expected a tensor on the GPU, got one on " + str(t.device))
+    if not t.is_contiguous():
+        raise ValueError("tree-layouts: expected a contiguous tensor (got a strided view)")
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def _kind(t, kind):
-    if kind is not None:
-        return kind
-    import torch
-    return {torch.int32: "i32", torch.int64: "i64"}[t.dtype]
+    """Key type name of t; an explicit kind must match t's element width."""
+    if kind is None:
+        import torch
+        names = {torch.int32: "i32", torch.int64: "i64", torch.uint32: "u32", torch.uint64: "u64"}
+        if t.dtype not in names:
+            raise ValueError(f"tree-layouts: no key type for tensor dtype {t.dtype}")
+        return names[t.dtype]
+    if kind not in WIDTH:
+        raise ValueError(f"tree-layouts: unknown key type {kind!r}")
+    if t.element_size() != WIDTH[kind]:
+        raise ValueError(f"tree-layouts: key type {kind} is {WIDTH[kind]} bytes wide but the "
+                         f"tensor's elements are {t.element_size()}")
+    return kind
+
+
+def _ans(ans, nq):
+    """Answer buffer: a contiguous 8-byte integer tensor of at least nq entries."""
+    if ans.element_size() != 8 or ans.is_floating_point():
+        raise ValueError("tree-layouts: answers are uint64, pass an int64 tensor")
+    if ans.numel() < nq:
+        raise ValueError(f"tree-layouts: {nq} queries need {nq} answers, the tensor holds "
+                         f"{ans.numel()}")
+    return _ptr(ans)
+
+
+def _queries(Q, kind):
+    if Q.element_size() != WIDTH[kind]:
+        raise ValueError(f"tree-layouts: queries must have the keys' width ({WIDTH[kind]} bytes)")
+    return _ptr(Q)
+
+
+def sync(stream=None):
+    """Wait for the stream and raise TLError(TL_ETIMEOUT) if an in-kernel
+    bounded wait of an untimed call gave up since the last report."""
+    _check(lib().tl_sync(_stream(stream)))
+
+
+def selftest_timeout(wait_ms, stream=None):
+    """Runs a kernel whose bounded wait can never be satisfied; returns
+    (status, elapsed ms).  The status must be TL_ETIMEOUT."""
+    out = ctypes.c_float(0.0)
+    st = lib().tl_selftest_timeout(wait_ms, _stream(stream), ctypes.byref(out))
+    return st, out.value
 
 
 def _ms(timed):
@@ -242,36 +294,36 @@ def unpermute(layout, A, b=16, kind=None, stream=None, timed=True):
 
 def query_bst(A, Q, ans, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
-    return _timed_call(lib().tl_query_bst, _ptr(A), A.numel(), DTYPE[k], _ptr(Q), Q.numel(),
-                       _ptr(ans), _stream(stream), timed=timed)
+    return _timed_call(lib().tl_query_bst, _ptr(A), A.numel(), DTYPE[k], _queries(Q, k), Q.numel(),
+                       _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def query_binary(A, Q, ans, kind=None, stream=None, timed=True):
     """Binary search of the sorted A: the key's index, its predecessor's
     index when absent, or n when every key is larger."""
     k = _kind(A, kind)
-    return _timed_call(lib().tl_query_binary, _ptr(A), A.numel(), DTYPE[k], _ptr(Q), Q.numel(),
-                       _ptr(ans), _stream(stream), timed=timed)
+    return _timed_call(lib().tl_query_binary, _ptr(A), A.numel(), DTYPE[k], _queries(Q, k), Q.numel(),
+                       _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def query_btree(A, b, Q, ans, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
-    return _timed_call(lib().tl_query_btree, _ptr(A), A.numel(), b, DTYPE[k], _ptr(Q),
-                       Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+    return _timed_call(lib().tl_query_btree, _ptr(A), A.numel(), b, DTYPE[k], _queries(Q, k),
+                       Q.numel(), _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def query_mveb(A, table, d, Q, ans, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_mveb, _ptr(A), _ptr(table), A.numel(), d, DTYPE[k],
-                       _ptr(Q), Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+                       _queries(Q, k), Q.numel(), _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def query_mveb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None, stream=None,
                           timed=True):
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_mveb_nonperfect, _ptr(A), _ptr(tables), _ptr(idx),
-                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _ptr(Q), Q.numel(),
-                       _ptr(ans), _stream(stream), timed=timed)
+                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _queries(Q, k), Q.numel(),
+                       _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def veb_tableset_device(n, device="cuda", stream=None, classic=False):
@@ -298,7 +350,7 @@ def query_veb(A, table, d, Q, ans, kind=None, stream=None, timed=True):
     """Classic vEB of a perfect tree, with a classic-rule table."""
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_veb, _ptr(A), _ptr(table), A.numel(), d, DTYPE[k],
-                       _ptr(Q), Q.numel(), _ptr(ans), _stream(stream), timed=timed)
+                       _queries(Q, k), Q.numel(), _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def query_veb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None, stream=None,
@@ -306,8 +358,8 @@ def query_veb_nonperfect(A, tables, idx, tidx, d, num_tables, Q, ans, kind=None,
     """Classic vEB of any size, with the classic table set."""
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_veb_nonperfect, _ptr(A), _ptr(tables), _ptr(idx),
-                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _ptr(Q), Q.numel(),
-                       _ptr(ans), _stream(stream), timed=timed)
+                       _ptr(tidx), A.numel(), d, num_tables, DTYPE[k], _queries(Q, k), Q.numel(),
+                       _ans(ans, Q.numel()), _stream(stream), timed=timed)
 
 
 def plan_export(layout, n, b=1, kind="u64", lds_bytes=0, inverse=False):
@@ -338,6 +390,18 @@ def learned_fit(sorted_keys, num_leaves=1 << 16, kind=None, stream=None, timed=T
     return model, ms
 
 
+def learned_predict(model, keys, kind=None, stream=None):
+    """(pred, err) int64 tensors: every key's predicted rank and its leaf's
+    error bound, from the device's own arithmetic (-1 for an empty leaf)."""
+    import torch
+    k = _kind(keys, kind)
+    pred = torch.empty(keys.numel(), dtype=torch.int64, device=keys.device)
+    err = torch.empty_like(pred)
+    _check(lib().tl_learned_predict(_ptr(model), _ptr(keys), keys.numel(), DTYPE[k], _ptr(pred),
+                                    _ptr(err), _stream(stream)))
+    return pred, err
+
+
 def learned_max_err(model):
     """largest leaf rank error (uint32 at byte offset 32 of the model)"""
     return int(model[4].item()) & 0xFFFFFFFF
@@ -346,7 +410,8 @@ def learned_max_err(model):
 def query_btree_learned(A, b, model, Q, ans, stats=None, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
     return _timed_call(lib().tl_query_btree_learned, _ptr(A), A.numel(), b, DTYPE[k], _ptr(model),
-                       _ptr(Q), Q.numel(), _ptr(ans), _ptr(stats), _stream(stream), timed=timed)
+                       _queries(Q, k), Q.numel(), _ans(ans, Q.numel()), _ptr(stats),
+                       _stream(stream), timed=timed)
 
 
 def bench_learned(A, keys, Q, plain_ans, b, steps, warmup, n, num_leaves=1 << 16):
