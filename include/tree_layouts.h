@@ -88,6 +88,24 @@ tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, 
 
 /* ---- inverse permutations: layout -> sorted order, in place ------------- */
 
+/* Allocation-free tl_permute_btree: the same layout, enqueued on `stream`,
+ * with every piece of scratch (plan tables, move flags, stash) taken from a
+ * fixed static device array of TL_NOALLOC_SCRATCH_BYTES per device instead
+ * of library allocations.  No counterpart in the reference; it is for
+ * callers that cannot allocate (code around graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the call makes no hipMalloc,
+ * hipFree or other allocating call.  Its plan keeps every stash within the
+ * array, so large n may take slower passes than tl_permute_btree.  Returns
+ * TL_EINVAL, before any launch, for n > TL_NOALLOC_MAX_KEYS or a plan whose
+ * scratch does not fit the array.  Every call on a device shares that
+ * device's array: calls that overlap on one device (two streams, two host
+ * threads) race and must be serialised.  A bounded wait that gives up is
+ * reported by the next tl_sync(). */
+#define TL_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define TL_NOALLOC_MAX_KEYS (1ull << 31)
+tl_status tl_permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                   void *stream);
+
 /* The exact inverses of tl_permute_{bst,btree,mveb} for the same (n, b): the
  * key at each slot of the layout goes back to its rank, whatever the values.
  * No counterpart in the reference, whose README (README.md:2) names the

@@ -29,6 +29,13 @@ def test_invalid_arguments_fail_without_device():
         tl_amd.plan_export("btree", 100, 0)
     with pytest.raises(tl_amd.TLError):
         tl_amd.plan_export("btree", 100, 5000)
+    # the allocation-free permutation rejects sizes past its limit before it
+    # touches the device (the pointer is never dereferenced)
+    L = tl_amd.lib()
+    st = L.tl_permute_btree_noalloc(1 << 20, tl_amd.TL_NOALLOC_MAX_KEYS + 1, 16,
+                                    tl_amd.DTYPE["u64"], None)
+    assert st == tl_amd.TL_EINVAL
+    assert "TL_NOALLOC_MAX_KEYS" in L.tl_last_error().decode()
 
 
 @pytest.mark.parametrize("layout,bs", [("btree", (16, 8, 2, 3, 32, 1)), ("bst", (1,)),

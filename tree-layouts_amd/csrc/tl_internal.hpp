@@ -56,7 +56,17 @@ struct Ctx {
     uint32_t *err_flag = nullptr;   // device word set by bounded waits on timeout
     uint64_t wall_khz = 100000;     // rate of the device wall clock
     uint64_t wait_ticks = 0;        // bound on one in-kernel wait, wall-clock ticks
+    // this device's copy of the static scratch array of the allocation-free
+    // entry point (TL_NOALLOC_SCRATCH_BYTES), as a device address
+    unsigned char *noalloc = nullptr;
 };
+
+// split of the allocation-free scratch: plan tables, then move flags (every
+// chained op its own region), then the stash
+constexpr size_t NOALLOC_BLOB_BYTES = 8u << 20;
+constexpr size_t NOALLOC_FLAG_BYTES = 4u << 20;
+constexpr size_t NOALLOC_STASH_BYTES =
+    (size_t)TL_NOALLOC_SCRATCH_BYTES - NOALLOC_BLOB_BYTES - NOALLOC_FLAG_BYTES;
 
 tl_status get_ctx(Ctx **out);
 // slot 0: stash of displaced keys, 1: move flags / counters, 2: plan tables,

@@ -869,6 +869,55 @@ static tl_status permute(Layout lay, void *dev_A, uint64_t n, uint64_t b, tl_dty
     return TL_OK;
 }
 
+// tl_permute_btree with all scratch in the device's static array (Ctx::noalloc):
+// the plan is built with every stash capped at the array's stash part, its
+// tables are copied into the array, its flags cleared by a fill kernel (every
+// chained op has a region of its own), and the launches take the array's
+// addresses.  No allocation, no memset.
+static tl_status permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype dt,
+                                       void *stream) {
+    uint32_t W;
+    TL_TRY(check_args(L_BTREE, dev_A, n, b, dt, W));
+    if (n > TL_NOALLOC_MAX_KEYS) {
+        set_error("tl_permute_btree_noalloc: n = " + std::to_string(n) +
+                  " exceeds TL_NOALLOC_MAX_KEYS = " + std::to_string(TL_NOALLOC_MAX_KEYS));
+        return TL_EINVAL;
+    }
+    Ctx *c;
+    TL_TRY(get_ctx(&c));
+    Plan pl;
+    build_plan(pl, tuning_for(c), L_BTREE, n, b, W, NOALLOC_STASH_BYTES / W);
+    std::string why;
+    if (!validate_plan(pl, n, why)) {
+        set_error(why);
+        return TL_EINVAL;
+    }
+    const size_t blob_bytes = pl.blob.size() * 8;
+    const size_t flag_bytes = plan_flag_bytes_all(pl);
+    const size_t stash_bytes = pl.max_stash * W;
+    if (blob_bytes > NOALLOC_BLOB_BYTES || flag_bytes > NOALLOC_FLAG_BYTES ||
+        stash_bytes > NOALLOC_STASH_BYTES) {
+        set_error("tl_permute_btree_noalloc: the plan for n = " + std::to_string(n) +
+                  " needs " + std::to_string(blob_bytes) + " table, " +
+                  std::to_string(flag_bytes) + " flag and " + std::to_string(stash_bytes) +
+                  " stash bytes; the static scratch holds " +
+                  std::to_string(NOALLOC_BLOB_BYTES) + ", " + std::to_string(NOALLOC_FLAG_BYTES) +
+                  " and " + std::to_string(NOALLOC_STASH_BYTES));
+        return TL_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t *blob = (uint64_t *)c->noalloc;
+    uint32_t *flags = (uint32_t *)(c->noalloc + NOALLOC_BLOB_BYTES);
+    void *stash = c->noalloc + NOALLOC_BLOB_BYTES + NOALLOC_FLAG_BYTES;
+    if (blob_bytes) {
+        // stream-ordered after earlier calls on `s`; the host vector dies here
+        TL_HIP(hipMemcpyAsync(blob, pl.blob.data(), blob_bytes, hipMemcpyHostToDevice, s));
+        TL_HIP(hipStreamSynchronize(s));
+    }
+    if (flag_bytes) TL_TRY(fill_u32_launch(flags, flag_bytes / 4, 0u, s));
+    return plan_launch(c, pl, dev_A, W, s, blob, stash, flags);
+}
+
 // Serialized plan: per op 32 words
 //   [type, nruns, nitems, p0..p7, ms.kind, ms.P, ms.L, ms.k, ms.r, ms.l, ms.CH, ms.nchunk,
 //    runs_at, 0...] followed by the blob.
@@ -957,6 +1006,11 @@ extern "C" tl_status tl_unpermute_btree(void *dev_A, uint64_t n, uint64_t b, tl_
 extern "C" tl_status tl_unpermute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
                                        float *ms) {
     return tl::permute(tl::L_MVEB, dev_A, n, 0, dtype, stream, ms, true);
+}
+
+extern "C" tl_status tl_permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                              void *stream) {
+    return tl::permute_btree_noalloc(dev_A, n, b, dtype, stream);
 }
 
 extern "C" tl_status tl_selftest_timeout(uint32_t wait_ms, void *stream, float *ms) {

@@ -809,7 +809,10 @@ inline void invert_plan(const Plan &f, Plan &g) {
     }
 }
 
-inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint64_t b, uint32_t W) {
+// stash_cap (elements, 0 = none): a hard bound on every stash the plan uses,
+// for callers whose scratch is fixed in advance
+inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint64_t b, uint32_t W,
+                       uint64_t stash_cap = 0) {
     // scratch stays at most n/16 keys (or 2^24 keys for small arrays)
     Tuning tu = tu0;
     tu.row_stash_max = tu.row_stash_env >= 0 ? (uint64_t)tu.row_stash_env
@@ -820,6 +823,11 @@ inline void build_plan(Plan &pl, const Tuning &tu0, Layout lay, uint64_t n, uint
         tu.walk_lanes = (uint64_t)tu.walk_env;
     } else {
         tu.walk_lanes = std::max<uint64_t>(tu.stash_max, n / 16);
+    }
+    if (stash_cap) {
+        tu.stash_max = std::min(tu.stash_max, stash_cap);
+        tu.row_stash_max = std::min(tu.row_stash_max, stash_cap);
+        tu.walk_lanes = std::min(tu.walk_lanes, stash_cap);
     }
     Planner P{pl, tu, W};
     if (lay == L_BST)

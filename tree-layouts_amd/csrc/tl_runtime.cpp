@@ -14,6 +14,14 @@ static thread_local std::string g_last_error;
 static std::mutex g_ctx_mu;
 static Ctx g_ctx[64];
 
+// Device memory the library holds without allocating, one copy per device
+// (zero at load): the error word of the bounded in-kernel waits, and the
+// scratch of the allocation-free permutation.  Host code uses these only
+// through the addresses hipGetSymbolAddress returns for the current device
+// (Ctx::err_flag, Ctx::noalloc); the names themselves are host stand-ins.
+static __device__ __align__(256) uint32_t err_words[64];
+static __device__ __align__(256) unsigned char noalloc_scratch[TL_NOALLOC_SCRATCH_BYTES];
+
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 tl_status get_ctx(Ctx **out) {
@@ -37,10 +45,14 @@ tl_status get_ctx(Ctx **out) {
         const char *w = getenv("TL_WAIT_MS");
         const uint64_t wait_ms = w ? std::max<uint64_t>(1, strtoull(w, nullptr, 10)) : 4000;
         c.wait_ticks = wait_ms * c.wall_khz;
+        // set-up allocates nothing: the device's own copies of the static
+        // arrays, looked up once with this device current
         void *p = nullptr;
-        TL_HIP(hipMalloc(&p, 256));
-        TL_HIP(hipMemset(p, 0, 256));
+        TL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(err_words)));
         c.err_flag = (uint32_t *)p;
+        void *q = nullptr;
+        TL_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(noalloc_scratch)));
+        c.noalloc = (unsigned char *)q;
         c.device = dev;
     }
     *out = &c;

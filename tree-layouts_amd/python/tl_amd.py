@@ -19,6 +19,9 @@ TL_OK, TL_EINVAL, TL_ENOMEM, TL_EHIP, TL_ETIMEOUT, TL_EUNSUPPORTED = range(6)
 DTYPE = {"u32": 0, "u64": 1, "i32": 2, "i64": 3}
 WIDTH = {"u32": 4, "u64": 8, "i32": 4, "i64": 8}
 LAYOUT = {"bst": 0, "btree": 1, "mveb": 2, "veb": 3}   # "veb": the classic (CPU) vEB
+# limits of tl_permute_btree_noalloc (include/tree_layouts.h)
+TL_NOALLOC_SCRATCH_BYTES = 64 << 20
+TL_NOALLOC_MAX_KEYS = 1 << 31
 
 
 class TLError(RuntimeError):
@@ -43,6 +46,8 @@ _SIGS = {
                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
     "tl_permute_mveb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                         ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_btree_noalloc": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                 ctypes.c_void_p],
     "tl_unpermute_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                          ctypes.POINTER(ctypes.c_float)],
     "tl_unpermute_btree": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
@@ -248,6 +253,14 @@ def permute_btree(A, b, kind=None, stream=None, timed=True):
     k = _kind(A, kind)
     return _timed_call(lib().tl_permute_btree, _ptr(A), A.numel(), b, DTYPE[k],
                        _stream(stream), timed=timed)
+
+
+def permute_btree_noalloc(A, b, kind=None, stream=None):
+    """tl_permute_btree_noalloc: the B-tree layout with all scratch in the
+    library's static device array (enqueued only; errors of in-kernel waits
+    surface at sync())."""
+    k = _kind(A, kind)
+    _check(lib().tl_permute_btree_noalloc(_ptr(A), A.numel(), b, DTYPE[k], _stream(stream)))
 
 
 def permute_mveb(A, kind=None, stream=None, timed=True):
