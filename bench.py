@@ -289,24 +289,21 @@ def run_batch_permutes(args, dev):
     print(json.dumps(measure_batch_permutes(args, dev, calibrate=True)), flush=True)
 
 
-def run_batch(args, dev):
-    """Uneven batch: 4096 sorted arrays, sizes floor(2^U), U ~ U[8, 22]
-    (seed 5), int32 / uint64 keys 50/50, layouts BST / B-tree b=8 / mvEB
-    round-robin, keys = cumsum of U{1..64} gaps seeded per array, 4 * size
-    uniform-hit queries per array, one permute and one query call per
-    array."""
+def config5_arrays(dev, count=4096, queries=True):
+    """The uneven-batch workload: `count` sorted arrays, sizes floor(2^U),
+    U ~ U[8, 22] (numpy seed 5), int32 / uint64 keys 50/50 (same generator),
+    layouts BST / B-tree b=8 / mvEB round robin, keys = cumsum of U{1..64}
+    gaps (torch CUDA generator seeded 5*1000003 + i), and with `queries`
+    4 * size uniform-hit queries per array.  Returns (arrays, pristine
+    keys, algorithmic permute bytes, algorithmic query bytes)."""
     import torch
-    import tl_amd
     rng = np.random.default_rng(5)
-    count = args.batch_arrays
     sizes = np.floor(2.0 ** rng.uniform(8, 22, size=count)).astype(np.int64)
     kinds = np.where(rng.random(count) < 0.5, "i32", "u64")
     layouts = ["bst", "btree", "mveb"]
     arrays, pristine = [], []
-    keys_total = 0
     perm_bytes = 0.0
     query_bytes = 0.0
-    log(f"[bench] generating {count} arrays, {int(sizes.sum())} keys")
     for i in range(count):
         n, kind, lay = int(sizes[i]), str(kinds[i]), layouts[i % 3]
         g = torch.Generator(device=dev)
@@ -314,14 +311,17 @@ def run_batch(args, dev):
         tdt = torch.int32 if kind == "i32" else torch.int64
         gaps = torch.randint(1, 65, (n,), generator=g, device=dev, dtype=torch.int64)
         keys = torch.cumsum(gaps, 0).to(tdt)
-        qi = torch.randint(0, n, (4 * n,), generator=g, device=dev, dtype=torch.int64)
-        Q = keys[qi]
-        del gaps, qi
+        Q = ans = None
+        if queries:
+            qi = torch.randint(0, n, (4 * n,), generator=g, device=dev, dtype=torch.int64)
+            Q = keys[qi]
+            ans = torch.empty(4 * n, dtype=torch.int64, device=dev)
+            del qi
+        del gaps
         w = 4 if kind == "i32" else 8
         arrays.append({"A": keys.clone(), "kind": kind, "layout": lay, "b": 8, "Q": Q,
-                       "ans": torch.empty(4 * n, dtype=torch.int64, device=dev)})
+                       "ans": ans})
         pristine.append(keys)
-        keys_total += n
         perm_bytes += 2.0 * n * w
         if lay == "btree":
             depth = 1
@@ -333,7 +333,20 @@ def run_batch(args, dev):
         else:
             depth = int(np.floor(np.log2(n))) + 1
             node = w
-        query_bytes += 4.0 * n * (w + 8 + depth * node)
+        if queries:
+            query_bytes += 4.0 * n * (w + 8 + depth * node)
+    return arrays, pristine, perm_bytes, query_bytes
+
+
+def run_batch(args, dev):
+    """Uneven batch (config5_arrays): one permute and query pass over the
+    whole batch per step, through tl_batch_run replayed from its HIP graph."""
+    import torch
+    import tl_amd
+    count = args.batch_arrays
+    log(f"[bench] generating {count} arrays")
+    arrays, pristine, perm_bytes, query_bytes = config5_arrays(dev, count)
+    keys_total = sum(p.numel() for p in pristine)
     torch.cuda.synchronize()
     # tl_batch_run: every plan built and uploaded at create time, the arrays
     # that fit in LDS in batched launches, one HIP graph replayed per step
