@@ -37,7 +37,8 @@ typedef enum {
     TL_ENOMEM = 2,      /* device scratch could not be allocated */
     TL_EHIP = 3,        /* a HIP call failed; see tl_last_error() */
     TL_ETIMEOUT = 4,    /* a bounded in-kernel wait gave up (never expected) */
-    TL_EUNSUPPORTED = 5 /* combination this build does not implement */
+    TL_EUNSUPPORTED = 5, /* combination this build does not implement */
+    TL_EIO = 6           /* a layout file could not be read or written, or is corrupt */
 } tl_status;
 
 typedef enum { TL_U32 = 0, TL_U64 = 1, TL_I32 = 2, TL_I64 = 3 } tl_dtype;
@@ -86,8 +87,6 @@ tl_status tl_permute_mveb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
  * timePermutevEB<T>(T *A, uint64_t n, uint32_t p)  (cpu/src/veb-cycles.h:381). */
 tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, float *ms);
 
-/* ---- inverse permutations: layout -> sorted order, in place ------------- */
-
 /* Allocation-free tl_permute_btree: the same layout, enqueued on `stream`,
  * with every piece of scratch (plan tables, move flags, stash) taken from a
  * fixed static device array of TL_NOALLOC_SCRATCH_BYTES per device instead
@@ -105,6 +104,8 @@ tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, 
 #define TL_NOALLOC_MAX_KEYS (1ull << 31)
 tl_status tl_permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
                                    void *stream);
+
+/* ---- inverse permutations: layout -> sorted order, in place ------------- */
 
 /* The exact inverses of tl_permute_{bst,btree,mveb} for the same (n, b): the
  * key at each slot of the layout goes back to its rank, whatever the values.
@@ -217,6 +218,52 @@ tl_status tl_learned_predict(const void *dev_model, const void *dev_keys, uint64
 tl_status tl_query_btree_learned(const void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
                                  const void *dev_model, const void *dev_q, uint64_t nq,
                                  uint64_t *dev_ans, uint64_t *dev_stats, void *stream, float *ms);
+
+/* ---- before the layouts: raw keys -> sorted distinct keys --------------- */
+
+/* Sorts dev_keys[0..n) ascending in the key type's order (signed for TL_I32 /
+ * TL_I64) and drops repeated keys; the distinct keys then fill
+ * dev_keys[0..*n_out) (host).  Waits for its work.  n < 2^32; scratch is one
+ * library-owned buffer of n keys plus counters.  No counterpart in the
+ * reference: its drivers build sorted distinct input themselves
+ * (cpu/src/common.h:33-40) and every layout assumes it. */
+tl_status tl_sort_dedupe(void *dev_keys, uint64_t n, tl_dtype dtype, uint64_t *n_out,
+                         void *stream, float *ms);
+
+/* ---- layout files (no counterpart: the reference has no serialisation) --- */
+
+/* layout codes of a file: 0 BST, 1 B-tree, 2 mvEB, 3 classic vEB, or sorted */
+#define TL_FILE_SORTED 255u
+
+typedef struct {
+    uint32_t layout;      /* 0..3 or TL_FILE_SORTED */
+    tl_dtype dtype;
+    uint64_t n;
+    uint64_t b;           /* keys per node (B-tree), 1 for a BST, else 0 */
+    uint32_t d;           /* levels of the binary tree (vEB layouts), floor(log2 n)+1 otherwise */
+    uint32_t num_tables;  /* vEB query table set: structures ... */
+    uint32_t table_size;  /* ... and table entries in all; 0 for the other layouts */
+    uint32_t version;
+    uint64_t checksum;    /* position-sensitive checksum of the keys */
+} tl_layout_info;
+
+/* Writes the n keys of dev_A (laid out as `layout`, with node width b for a
+ * B-tree) to `path`: a 256-byte header {magic "TLLAYOUT", version, layout,
+ * dtype, n, b, d, table-set shape, offsets, checksum}, the vEB query table
+ * set for layouts 2 and 3 (tl_veb_table entries, idx, table_idx), and the raw
+ * keys at a 4096-aligned offset.  Waits for its work. */
+tl_status tl_layout_save(const char *path, const void *dev_A, uint64_t n, uint32_t layout,
+                         uint64_t b, tl_dtype dtype, void *stream);
+/* Reads and checks the header only (to size the buffers of a load). */
+tl_status tl_layout_read_info(const char *path, tl_layout_info *info);
+/* Loads the keys into dev_A (capacity keys, >= n) and, when the pointers are
+ * non-NULL, the table set into dev_tables (>= table_size entries), dev_idx
+ * and dev_table_idx (>= num_tables entries): ready for tl_query_mveb_nonperfect
+ * / tl_query_veb_nonperfect.  Recomputes the checksum on the device and
+ * returns TL_EIO on a mismatch.  Waits for its work. */
+tl_status tl_layout_load(const char *path, void *dev_A, uint64_t capacity,
+                         tl_veb_table *dev_tables, uint64_t *dev_idx, uint32_t *dev_table_idx,
+                         tl_layout_info *info, void *stream);
 
 /* ---- uneven batch (no counterpart in the reference) --------------------- */
 

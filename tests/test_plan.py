@@ -238,3 +238,17 @@ def test_rotation_lane_walk_model():
                     A, writes = _walk_model(L, c, w, K)
                     assert writes == [1] * L, (L, c, w, K)
                     assert A == list(np.roll(np.arange(L), c)), (L, c, w, K)
+
+
+def test_layout_file_header_errors_without_device(tmp_path):
+    """tl_layout_read_info checks the header on the host: a missing file, a
+    file that is not a layout file, and a truncated header are TL_EIO"""
+    L = tl_amd.lib()
+    info = tl_amd.LayoutInfo()
+    p = os.path.join(str(tmp_path), "x.tl")
+    assert L.tl_layout_read_info(p.encode(), info) == tl_amd.TL_EIO
+    open(p, "wb").write(b"NOTALAYOUT" + bytes(300))
+    assert L.tl_layout_read_info(p.encode(), info) == tl_amd.TL_EIO
+    assert "not a layout file" in L.tl_last_error().decode()
+    open(p, "wb").write(b"TLLAYOUT" + bytes(20))
+    assert L.tl_layout_read_info(p.encode(), info) == tl_amd.TL_EIO

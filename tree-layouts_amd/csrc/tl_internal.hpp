@@ -51,8 +51,8 @@ struct Ctx {
     int num_cus = 256;
     size_t lds_max = 65536;   // dynamic LDS a workgroup may request
     // grow-only scratch areas (device memory), one per purpose
-    void *scratch[4] = {nullptr, nullptr, nullptr, nullptr};
-    size_t scratch_bytes[4] = {0, 0, 0, 0};
+    void *scratch[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t scratch_bytes[6] = {0, 0, 0, 0, 0, 0};
     uint32_t *err_flag = nullptr;   // device word set by bounded waits on timeout
     uint64_t wall_khz = 100000;     // rate of the device wall clock
     uint64_t wait_ticks = 0;        // bound on one in-kernel wait, wall-clock ticks
@@ -70,7 +70,7 @@ constexpr size_t NOALLOC_STASH_BYTES =
 
 tl_status get_ctx(Ctx **out);
 // slot 0: stash of displaced keys, 1: move flags / counters, 2: plan tables,
-// 3: miscellaneous
+// 3: miscellaneous, 4: the sort's second key buffer, 5: the sort's counters
 tl_status scratch(Ctx *c, int slot, size_t bytes, void **out);
 tl_status copy_to_device(Ctx *c, int slot, const void *host, size_t bytes, hipStream_t s,
                          void **dev_out);

@@ -15,7 +15,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtl_amd.so")
 
-TL_OK, TL_EINVAL, TL_ENOMEM, TL_EHIP, TL_ETIMEOUT, TL_EUNSUPPORTED = range(6)
+TL_OK, TL_EINVAL, TL_ENOMEM, TL_EHIP, TL_ETIMEOUT, TL_EUNSUPPORTED, TL_EIO = range(7)
+TL_FILE_SORTED = 255
 DTYPE = {"u32": 0, "u64": 1, "i32": 2, "i64": 3}
 WIDTH = {"u32": 4, "u64": 8, "i32": 4, "i64": 8}
 LAYOUT = {"bst": 0, "btree": 1, "mveb": 2, "veb": 3}   # "veb": the classic (CPU) vEB
@@ -32,6 +33,13 @@ class TLError(RuntimeError):
 
 class VebTable(ctypes.Structure):
     _fields_ = [("L", ctypes.c_uint64), ("R", ctypes.c_uint64), ("D", ctypes.c_uint32)]
+
+
+class LayoutInfo(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_uint32), ("dtype", ctypes.c_int), ("n", ctypes.c_uint64),
+                ("b", ctypes.c_uint64), ("d", ctypes.c_uint32), ("num_tables", ctypes.c_uint32),
+                ("table_size", ctypes.c_uint32), ("version", ctypes.c_uint32),
+                ("checksum", ctypes.c_uint64)]
 
 
 _lib = None
@@ -95,6 +103,15 @@ _SIGS = {
                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
                        ctypes.POINTER(ctypes.c_uint64)],
     "tl_sync": [ctypes.c_void_p],
+    "tl_sort_dedupe": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p,
+                       ctypes.POINTER(ctypes.c_float)],
+    "tl_layout_save": [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                       ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p],
+    "tl_layout_read_info": [ctypes.c_char_p, ctypes.POINTER(LayoutInfo)],
+    "tl_layout_load": [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(LayoutInfo),
+                       ctypes.c_void_p],
     "tl_selftest_timeout": [ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
 }
 class BatchItem(ctypes.Structure):
@@ -301,6 +318,47 @@ def unpermute(layout, A, b=16, kind=None, stream=None, timed=True):
                            timed=timed)
     return _timed_call(L.tl_unpermute_mveb, _ptr(A), A.numel(), DTYPE[k], _stream(stream),
                        timed=timed)
+
+
+# ---- raw keys -> sorted distinct keys; layout files --------------------------
+
+def sort_dedupe(A, kind=None, stream=None):
+    """Sorts A in place (the key type's order) and drops repeats; returns
+    (number of distinct keys, which fill A[:m], ms)."""
+    k = _kind(A, kind)
+    m = ctypes.c_uint64(0)
+    ms = ctypes.c_float(0.0)
+    _check(lib().tl_sort_dedupe(_ptr(A), A.numel(), DTYPE[k], ctypes.byref(m), _stream(stream),
+                                ctypes.byref(ms)))
+    return m.value, ms.value
+
+
+def _info_dict(info):
+    return {f: getattr(info, f) for f, _ in LayoutInfo._fields_}
+
+
+def layout_save(path, A, layout, b=0, kind=None, stream=None):
+    """Writes A (laid out as `layout`: 'bst', 'btree', 'mveb', 'veb' or
+    'sorted') to a layout file."""
+    k = _kind(A, kind)
+    code = TL_FILE_SORTED if layout == "sorted" else LAYOUT[layout]
+    _check(lib().tl_layout_save(os.fsencode(path), _ptr(A), A.numel(), code, b, DTYPE[k],
+                                _stream(stream)))
+
+
+def layout_read_info(path):
+    info = LayoutInfo()
+    _check(lib().tl_layout_read_info(os.fsencode(path), ctypes.byref(info)))
+    return _info_dict(info)
+
+
+def layout_load(path, A, tables=None, idx=None, tidx=None, stream=None):
+    """Loads a layout file into A (capacity A.numel()) and, if given, its vEB
+    table set into tables / idx / tidx; returns the header as a dict."""
+    info = LayoutInfo()
+    _check(lib().tl_layout_load(os.fsencode(path), _ptr(A), A.numel(), _ptr(tables), _ptr(idx),
+                                _ptr(tidx), ctypes.byref(info), _stream(stream)))
+    return _info_dict(info)
 
 
 # ---- queries ---------------------------------------------------------------
