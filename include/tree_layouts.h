@@ -284,10 +284,9 @@ typedef struct {
 
 /* lds_bytes: LDS budget of the in-LDS sub-layouts (0 = the device's whole
  * budget, 160 KiB less a 2 KiB reserve on gfx950);
- * wg_per_cu: persistent workgroups per compute unit (0 = 4);
- * wait_ms: bound on any single in-kernel wait of the persistent launch
- * (0 = the library default, 4000 ms or $TL_WAIT_MS); a wait that reaches it
- * makes the run report TL_ETIMEOUT. */
+ * wg_per_cu, wait_ms: workgroups per compute unit and the bound on one
+ * in-kernel wait of the single persistent launch, which is switched off in
+ * this build (tl_batch_run runs the planned path below); both are ignored. */
 typedef struct {
     uint32_t lds_bytes;
     uint32_t wg_per_cu;
@@ -298,8 +297,11 @@ typedef struct {
  * uploads the task list; cfg may be NULL. */
 tl_status tl_batch_create(const tl_batch_item *items, uint32_t count, const tl_batch_config *cfg,
                           void **handle);
-/* Permutes and queries every array of the batch in one persistent launch
- * (use_graph != 0: captured once in a HIP graph and replayed). */
+/* Permutes and queries every array of the batch by the planned path: the
+ * arrays that fit in LDS in batched one-workgroup-per-array launches, every
+ * other array's plan by waves of multi-array launches, then every array's
+ * queries; use_graph != 0 captures all of it once in a HIP graph and replays
+ * it.  (The single persistent work-queue launch is switched off.) */
 tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms);
 /* The same work through one tl_permute_* and one tl_query_* call per array. */
 tl_status tl_batch_run_separate(void *handle, void *stream, float *ms);

@@ -40,7 +40,7 @@ struct BOp {
     uint64_t ntasks;
     int64_t prev;                     // op that must complete first, -1 if none
     uint64_t flags_at;                // chained moves: first flag word
-    uint64_t stash_at;                // copy ops: element offset in the stash area
+    uint64_t stash_at;                // copy ops: offset of the array's stash, in 8-byte words
     // queries
     uint64_t Q, nq, ans, n, b;
     uint64_t tbl_at;                  // mvEB: tables (3 words each) in the blob
@@ -230,7 +230,8 @@ template <typename T>
 __device__ void ex_copy(const BOp &op, uint64_t sub, T *stash) {
     T *A = (T *)op.A;
     const uint64_t P = op.p[0], stride = op.p[1], count = op.p[2];
-    T *st = stash + op.stash_at + op.p[4];
+    // the array's region starts at an 8-byte word offset whatever its key width
+    T *st = (T *)((uint64_t *)stash + op.stash_at) + op.p[4];
     const uint64_t lo = sub * COPY_CHUNK, hi = min(count, lo + COPY_CHUNK);
     for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         if (op.p[3])
@@ -263,7 +264,7 @@ __device__ void ex_chained(const BOp &op, uint64_t t, uint32_t *flagbase, uint32
         wait_deps(op.ms, t, dst, len, flags, err, limit);
     }
     __syncthreads();
-    store_move<T, MOVE_PER>(A, stash + op.stash_at, op.ms, src, dst, len, v);
+    store_move<T, MOVE_PER>(A, (T *)((uint64_t *)stash + op.stash_at), op.ms, src, dst, len, v);
 }
 
 // ---- queries: QCHUNK queries per sub-task --------------------------------
