@@ -610,6 +610,21 @@ def main():
         f"{wall_ms / args.steps:.3f})")
     unperm_ms = time_unpermute(args, A, "btree", b, stream)
     log(f"[bench] unpermute {unperm_ms:.3f} ms/step")
+    # the same layout from the allocation-free entry point, whose plan keeps
+    # every stash within its fixed 64 MiB device array (so the scratch does not
+    # grow with n); the permutation only depends on (n, b), so repeating it on
+    # the same buffer repeats the same data movement
+    noalloc = None
+    if n <= tl_amd.TL_NOALLOC_MAX_KEYS:
+        tl_amd.permute_btree_noalloc(A, b, kind="u64")
+        na_total, na_per = event_steps(lambda: tl_amd.permute_btree_noalloc(A, b, kind="u64"),
+                                       max(1, args.steps // 2), stream)
+        tl_amd.sync()
+        na_ms = na_total / max(1, args.steps // 2)
+        noalloc = {"ms": round(na_ms, 4), "Gkeys_s": round(n / (na_ms * 1e-3) / 1e9, 3),
+                   "scratch_bytes": tl_amd.TL_NOALLOC_SCRATCH_BYTES,
+                   "algorithmic_frac": round(2.0 * n * 8 / (na_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"[bench] allocation-free permute {na_ms:.3f} ms/step")
 
     # ---- rebuild the true layout for the queries
     A.copy_(keys)
@@ -654,13 +669,24 @@ def main():
     # query roofline: algorithmic bytes of the DRAM-resident levels
     q_all_b, q_dram_b = btree_query_bytes(n, b, 8)
 
-    def qroof(ms):
-        ach = nq * q_dram_b / (ms * 1e-3) / 1e9
-        return {"bound": "hbm", "bytes_per_query": round(q_dram_b, 1),
-                "bytes_per_query_all_levels": round(q_all_b, 1), "achieved": round(ach, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+    # measured traffic per call (FETCH_SIZE + WRITE_SIZE, committed PMC summary;
+    # L2 misses, which include lines the Infinity Cache serves) at this size
+    q_traffic = None
+    if n == 1 << 30 and b == 16 and nq == 1 << 27:
+        q_traffic = measured_traffic_entry("btree_b16_n2p30_u64_query_traffic.txt")
 
-    q_traffic = measured_traffic_entry("btree_b16_n2p30_u64_query_traffic.txt")
+    def qroof(ms, tag=None):
+        ach = nq * q_dram_b / (ms * 1e-3) / 1e9
+        out = {"bound": "hbm", "bytes_per_query": round(q_dram_b, 1),
+               "bytes_per_query_all_levels": round(q_all_b, 1), "achieved": round(ach, 1),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+               "traffic": None}
+        if q_traffic and tag and f"{tag}_read_GB" in q_traffic:
+            t = (q_traffic[f"{tag}_read_GB"] + q_traffic[f"{tag}_write_GB"]) * 1e9
+            out["traffic"] = round(t)
+            out["traffic_GBs"] = round(t / (ms * 1e-3) / 1e9, 1)
+            out["traffic_frac"] = round(t / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        return out
 
     # ---- the uneven batch's permutations (4096 arrays, one graph replay a step)
     batch = None
@@ -684,16 +710,15 @@ def main():
         traffic = measured_traffic("btree_b16_n2p30_u64_permute_traffic.txt")
     queries = {"plain_ms": round(q_ms, 4), "plain_Mq_s": round(nq / q_ms / 1e3, 2),
                "plain_timings": timing_stats(nq, q_per, 1e-6),
-               "plain_roofline": qroof(q_ms),
+               "plain_roofline": qroof(q_ms, "plain"),
                "learned": learned,
                "binary_search": {"ms": round(bs_ms, 4), "Mq_s": round(nq / bs_ms / 1e3, 2),
-                                 "answers_ok": bs_ok}}
+                                 "answers_ok": bs_ok, "roofline": qroof(bs_ms, "binary")}}
     if learned is not None:
-        queries["learned_roofline"] = qroof(learned["ms"])
+        queries["learned_roofline"] = qroof(
+            learned["ms"], "learned16" if args.log2leaves == 16 else None)
     if learned_big is not None:
         queries["learned_more_leaves"] = learned_big
-    if q_traffic is not None:
-        queries["measured_traffic"] = q_traffic
     out = {
         "metric": "Gkeys/s permute + Mqueries/s, n=2^30 u64; learned vs plain start, exact answers",
         "value": round(n / (perm_ms * 1e-3) / 1e9, 3),
@@ -712,6 +737,7 @@ def main():
                    "n": n, "b": b, "queries": nq},
         "permute_timings": perm_stats,
         "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
+        "permute_fixed_scratch": noalloc,
         "queries": queries,
         "batch_permutes": batch,
         "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},

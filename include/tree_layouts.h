@@ -291,6 +291,11 @@ typedef struct {
     uint32_t lds_bytes;
     uint32_t wg_per_cu;
     uint32_t wait_ms;
+    uint32_t tile_k;          /* bit-reversal tiles of 2^k x 2^k keys: 5 or 6 (0 = 5) */
+    uint32_t stream_threads;  /* threads per streamed-gather partition: 256, 512 or
+                                 1024 (0 = $TL_STREAM_THREADS, else 1024) */
+    uint32_t btree_qpg;       /* B-tree descents per lane group: 2 or 4 (0 = 4) */
+    uint32_t veb_qpl;         /* mvEB queries per lane: 1, 2 or 4 (0 = 2) */
 } tl_batch_config;
 
 /* Plans every array (the same plans as the single-array permutations) and
@@ -307,9 +312,14 @@ tl_status tl_batch_run(void *handle, void *stream, int use_graph, float *ms);
 tl_status tl_batch_run_separate(void *handle, void *stream, float *ms);
 /* Launch plan size: number of ops, of sub-tasks and the LDS bytes per workgroup. */
 tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks, uint64_t *smem_bytes);
-/* Times candidate LDS budgets (32, 64, 96 and 160 KiB) on device copies of up
- * to `sample` items of the batch and returns the fastest configuration in
- * *best (wg_per_cu 4, wait_ms 0). */
+/* Autotuner: times configurations of the planned path on device copies of up
+ * to `sample` items of the batch (restored before every run) and returns the
+ * fastest in *best.  Coordinate search, one knob at a time from the defaults:
+ * the LDS budget of the plans (32, 64, 96, 160 KiB), the bit-reversal tile
+ * (32 or 64 keys a side), the threads per streamed-gather partition (256,
+ * 512, 1024: the keys each thread moves), the B-tree descents per lane group
+ * (2, 4) and the mvEB queries per lane (1, 2, 4).  B-tree arrays keep the b
+ * they were given: b is part of the layout the caller asked for. */
 tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
                             void *stream, tl_batch_config *best, float *best_ms);
 tl_status tl_batch_destroy(void *handle);

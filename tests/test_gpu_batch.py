@@ -59,7 +59,8 @@ def restore(arrays, host, dev):
 
 PERSISTENT_OFF = ("the single persistent launch stalled on gfx950 (twice, the second time with "
                   "wall-clock-bounded waits) and is disabled until its cause is shown; the "
-                  "sub-task list is checked by the host replay in test_batch_plan.py")
+                  "sub-task list is checked by the host replay in test_batch_plan.py and the "
+                  "executors by the CPU emulation in test_batch_emulated.py")
 
 
 @pytest.mark.skip(reason=PERSISTENT_OFF)
@@ -137,23 +138,37 @@ def test_batch_schedule_with_single_ops(knobs, dev, monkeypatch):
 
 
 def test_batch_planned_path_autotune(dev):
+    """The autotuner returns a configuration inside its search space, and the
+    batch built with it matches the oracle (eagerly and from its graph)."""
     arrays, host = make_batch(90, 6, 16, 7, dev)
     cfg = tl_amd.batch_autotune(arrays, sample=30)
-    assert cfg["lds_bytes"] in (32768, 65536, 98304, 163840) and cfg["wg_per_cu"] == 4
-    B = tl_amd.Batch(arrays, lds_bytes=cfg["lds_bytes"], wg_per_cu=cfg["wg_per_cu"])
+    assert cfg["lds_bytes"] in (0, 32768, 65536, 98304, 163840)
+    assert cfg["tile_k"] in (5, 6) and cfg["stream_threads"] in (256, 512, 1024)
+    assert cfg["btree_qpg"] in (2, 4) and cfg["veb_qpl"] in (1, 2, 4)
+    assert 0 < cfg["sample_ms"] < 1e29
+    B = tl_amd.Batch(arrays, **{k: cfg[k] for k in tl_amd.BATCH_KNOBS})
+    B.run(graph=False)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    restore(arrays, host, dev)
+    torch.cuda.synchronize()
     B.run(graph=True)
     torch.cuda.synchronize()
     check(arrays, host)
     B.close()
 
 
-@pytest.mark.skip(reason=PERSISTENT_OFF)
-def test_batch_autotune_returns_a_config(dev):
-    arrays, host = make_batch(90, 6, 16, 7, dev)
-    cfg = tl_amd.batch_autotune(arrays, sample=30)
-    assert cfg["lds_bytes"] in (32768, 65536, 98304, 163840) and cfg["wg_per_cu"] == 4
-    B = tl_amd.Batch(arrays, lds_bytes=cfg["lds_bytes"], wg_per_cu=cfg["wg_per_cu"])
+@pytest.mark.parametrize("knobs", [{"tile_k": 6, "stream_threads": 256, "btree_qpg": 2,
+                                    "veb_qpl": 1},
+                                   {"tile_k": 5, "stream_threads": 512, "btree_qpg": 4,
+                                    "veb_qpl": 4, "lds_bytes": 32768}])
+def test_batch_knobs_keep_results(knobs, dev):
+    """Every knob the autotuner turns leaves layouts and answers unchanged."""
+    arrays, host = make_batch(120, 4, 18, 13, dev)
+    B = tl_amd.Batch(arrays, **knobs)
     B.run(graph=True)
     torch.cuda.synchronize()
     check(arrays, host)
     B.close()
+
+

@@ -22,11 +22,14 @@
 #include <memory>
 #include <vector>
 
+#include "tl_descent.hpp"
 #include "tl_internal.hpp"
 #include "tl_kernels.hpp"
 #include "tl_plan.hpp"
 
 namespace tl {
+
+int &veb_qpl_fixed();   // tl_query.hip
 
 enum : uint32_t { OP_QUERY = 100 };
 constexpr uint32_t BATCH_THREADS = 256;
@@ -799,6 +802,7 @@ static tl_status upload_batch(Batch &B) {
 
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
+    if (B.cfg.tile_k == 5 || B.cfg.tile_k == 6) tu.bitrev_k = B.cfg.tile_k;
     if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;
     auto put = [&data](const void *src, size_t bytes) -> uint64_t {
@@ -910,9 +914,11 @@ static tl_status build_planned(Batch &B, Ctx *c) {
     return TL_OK;
 }
 
-// threads per workgroup of the batch's streamed gathers:
-// $TL_STREAM_THREADS (256, 512 or 1024), else 1024
-static uint32_t stream_threads() {
+// threads per workgroup of the batch's streamed gathers: the configuration's
+// (256, 512 or 1024), else $TL_STREAM_THREADS, else 1024
+static uint32_t stream_threads(const Batch &B) {
+    const uint32_t c = B.cfg.stream_threads;
+    if (c == 256 || c == 512 || c == 1024) return c;
     const char *e = getenv("TL_STREAM_THREADS");
     const int v = e ? atoi(e) : 1024;
     return (v == 256 || v == 512) ? (uint32_t)v : 1024u;
@@ -1031,7 +1037,7 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 const MStream d{A, runs, (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
                                 (uint32_t)op.p[2], (uint32_t)op.p[3], 0};
                 const bool fused = op.p[3] != 0;
-                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, stream_threads(), &d, sizeof d,
+                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, stream_threads(B), &d, sizeof d,
                     op.nitems, fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
                 break;
             }
@@ -1195,7 +1201,22 @@ static tl_status run_planned(Batch &B, Ctx *c, hipStream_t s) {
 }
 
 // every array's query batch, once every layout is in place
+// query shapes of the configuration, fixed on this host thread while the
+// batch's query launches are issued (or captured)
+struct QueryShape {
+    int qpg, qpl;
+    QueryShape(const tl_batch_config &c) : qpg(btree_qpg_fixed()), qpl(veb_qpl_fixed()) {
+        btree_qpg_fixed() = (int)c.btree_qpg;
+        veb_qpl_fixed() = (int)c.veb_qpl;
+    }
+    ~QueryShape() {
+        btree_qpg_fixed() = qpg;
+        veb_qpl_fixed() = qpl;
+    }
+};
+
 static tl_status run_queries(Batch &B, const unsigned char *base, hipStream_t s) {
+    QueryShape shape(B.cfg);
     TL_TRY(fork_branches(B, 1, s));
     for (uint32_t a = 0; a < B.items.size(); a++) {
         const tl_batch_item &it = B.items[a];
@@ -1336,14 +1357,19 @@ extern "C" tl_status tl_batch_run(void *handle, void *stream, int use_graph, flo
     return TL_OK;
 }
 
-// the comparison path: every array through the single-array entry points
+// the comparison path: every array through the single-array entry points, one
+// permute call and one query call per array; the mvEB query tables are the
+// ones uploaded when the batch was created, so the timed region holds only
+// the calls themselves
 extern "C" tl_status tl_batch_run_separate(void *handle, void *stream, float *ms) {
     if (!handle) return TL_EINVAL;
     Batch *B = (Batch *)handle;
     hipStream_t s = (hipStream_t)stream;
+    const unsigned char *base = (const unsigned char *)B->d_pdata;
     Timer tm;
     TL_TRY(tm.start(s, ms));
-    for (const tl_batch_item &it : B->items) {
+    for (uint32_t a = 0; a < B->items.size(); a++) {
+        const tl_batch_item &it = B->items[a];
         if (it.layout == 0)
             TL_TRY(tl_permute_bst(it.A, it.n, it.dtype, stream, nullptr));
         else if (it.layout == 1)
@@ -1351,32 +1377,17 @@ extern "C" tl_status tl_batch_run_separate(void *handle, void *stream, float *ms
         else
             TL_TRY(tl_permute_mveb(it.A, it.n, it.dtype, stream, nullptr));
         if (!it.nq) continue;
-        if (it.layout == 0)
+        if (it.layout == 0) {
             TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
-        else if (it.layout == 1)
+        } else if (it.layout == 1) {
             TL_TRY(tl_query_btree(it.A, it.n, it.b, it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
-        else {
-            std::vector<uint64_t> tb, ix, tix;
-            host_tableset(it.n, tb, ix, tix);
-            // device copies of the table set, in the reference's packed format
-            std::vector<tl_veb_table> t(tb.size() / 3);
-            for (size_t k = 0; k < t.size(); k++)
-                t[k] = tl_veb_table{tb[3 * k], tb[3 * k + 1], (uint32_t)tb[3 * k + 2]};
-            std::vector<uint32_t> t32(tix.begin(), tix.end());
-            void *dt, *di, *dti;
-            TL_HIP(hipMalloc(&dt, t.size() * sizeof(tl_veb_table)));
-            TL_HIP(hipMalloc(&di, ix.size() * 8));
-            TL_HIP(hipMalloc(&dti, t32.size() * 4));
-            TL_HIP(hipMemcpy(dt, t.data(), t.size() * sizeof(tl_veb_table), hipMemcpyHostToDevice));
-            TL_HIP(hipMemcpy(di, ix.data(), ix.size() * 8, hipMemcpyHostToDevice));
-            TL_HIP(hipMemcpy(dti, t32.data(), t32.size() * 4, hipMemcpyHostToDevice));
-            TL_TRY(tl_query_mveb_nonperfect(it.A, (const tl_veb_table *)dt, (const uint64_t *)di,
-                                            (const uint32_t *)dti, it.n, 0, (uint32_t)ix.size(),
+        } else {
+            const QTables &q = B->qt[a];
+            const uint32_t d = floor_log2_64(it.n ? it.n : 1) + 1;
+            TL_TRY(tl_query_mveb_nonperfect(it.A, (const tl_veb_table *)(base + q.tables_at),
+                                            (const uint64_t *)(base + q.idx_at),
+                                            (const uint32_t *)(base + q.tidx_at), it.n, d, q.nt,
                                             it.dtype, it.Q, it.nq, it.ans, stream, nullptr));
-            TL_HIP(hipStreamSynchronize(s));
-            (void)hipFree(dt);
-            (void)hipFree(di);
-            (void)hipFree(dti);
         }
     }
     return tm.stop();
@@ -1387,9 +1398,9 @@ extern "C" tl_status tl_batch_destroy(void *handle) {
     return TL_OK;
 }
 
-// autotuner: candidate LDS budgets and persistent grid sizes, timed on device
-// copies of a sample of the batch (restored from pristine copies before every
-// run, so every candidate permutes sorted input)
+// autotuner: coordinate search over the planned path's knobs, each candidate
+// timed on device copies of a sample of the batch (restored from pristine
+// copies before every run, so every candidate permutes sorted input)
 extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
                                        void *stream, tl_batch_config *best, float *best_ms) {
     if (!items || !count || !best) {
@@ -1401,14 +1412,26 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
     const uint32_t step = std::max<uint32_t>(1, count / m);
     std::vector<tl_batch_item> pick;
     std::vector<void *> pristine, work, answers;
+    auto release = [&]() {
+        for (void *p : pristine) (void)hipFree(p);
+        for (void *p : work) (void)hipFree(p);
+        for (void *p : answers) (void)hipFree(p);
+    };
     for (uint32_t i = 0; i < count && pick.size() < m; i += step) {
         tl_batch_item it = items[i];
         const size_t bytes = it.n * ((it.dtype == TL_U32 || it.dtype == TL_I32) ? 4 : 8);
         void *p = nullptr, *w = nullptr, *a = nullptr;
-        TL_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8)));
-        TL_HIP(hipMalloc(&w, std::max<size_t>(bytes, 8)));
-        TL_HIP(hipMalloc(&a, std::max<size_t>(it.nq * 8, 8)));
-        TL_HIP(hipMemcpy(p, it.A, bytes, hipMemcpyDeviceToDevice));
+        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess ||
+            hipMalloc(&w, std::max<size_t>(bytes, 8)) != hipSuccess ||
+            hipMalloc(&a, std::max<size_t>(it.nq * 8, 8)) != hipSuccess ||
+            hipMemcpy(p, it.A, bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            for (void *q : {p, w, a})
+                if (q) (void)hipFree(q);
+            release();
+            set_error("tl_batch_autotune: no memory for the sample");
+            return TL_ENOMEM;
+        }
         pristine.push_back(p);
         work.push_back(w);
         answers.push_back(a);
@@ -1416,14 +1439,10 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
         it.ans = (uint64_t *)a;
         pick.push_back(it);
     }
-    float best_t = 1e30f;
-    tl_status result = TL_OK;
-    // runs take the planned path, whose only knob is the LDS budget of the
-    // plans (the tile of whole-array LDS permutes and the run packing)
-    for (uint32_t lds : {32768u, 65536u, 98304u, 163840u}) {
-        tl_batch_config cfg{lds, 4, 0};
+    // mean of two timed runs after a warm-up run; +inf when the candidate fails
+    auto time_cfg = [&](const tl_batch_config &cfg) -> float {
         void *h = nullptr;
-        if (tl_batch_create(pick.data(), (uint32_t)pick.size(), &cfg, &h) != TL_OK) continue;
+        if (tl_batch_create(pick.data(), (uint32_t)pick.size(), &cfg, &h) != TL_OK) return 1e30f;
         float t = 0.f, tsum = 0.f;
         bool ok = true;
         for (int rep = 0; rep < 3 && ok; rep++) {
@@ -1438,20 +1457,44 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
             if (rep) tsum += t;
         }
         tl_batch_destroy(h);
-        if (ok && tsum / 2 < best_t) {
-            best_t = tsum / 2;
-            *best = cfg;
+        return ok ? tsum / 2 : 1e30f;
+    };
+    tl_batch_config cur{};
+    cur.lds_bytes = 0;
+    cur.wg_per_cu = 4;
+    cur.tile_k = 5;
+    cur.stream_threads = 1024;
+    cur.btree_qpg = 4;
+    cur.veb_qpl = 2;
+    float best_t = time_cfg(cur);
+    const std::vector<std::pair<uint32_t tl_batch_config::*, std::vector<uint32_t>>> knobs = {
+        {&tl_batch_config::lds_bytes, {32768u, 65536u, 98304u, 163840u}},
+        {&tl_batch_config::tile_k, {5u, 6u}},
+        {&tl_batch_config::stream_threads, {256u, 512u, 1024u}},
+        {&tl_batch_config::btree_qpg, {2u, 4u}},
+        {&tl_batch_config::veb_qpl, {1u, 2u, 4u}},
+    };
+    for (const auto &kv : knobs) {
+        for (uint32_t v : kv.second) {
+            if (cur.*(kv.first) == v) continue;
+            tl_batch_config cand = cur;
+            cand.*(kv.first) = v;
+            const float t = time_cfg(cand);
+            if (t < best_t) {
+                best_t = t;
+                cur = cand;
+            }
         }
     }
-    for (void *p : pristine) (void)hipFree(p);
-    for (void *p : work) (void)hipFree(p);
-    for (void *p : answers) (void)hipFree(p);
+    release();
     if (best_t >= 1e30f) {
         set_error("tl_batch_autotune: no candidate ran");
-        result = TL_EUNSUPPORTED;
+        if (best_ms) *best_ms = best_t;
+        return TL_EUNSUPPORTED;
     }
+    *best = cur;
     if (best_ms) *best_ms = best_t;
-    return result;
+    return TL_OK;
 }
 
 // host-only dry run of tl_batch_create (no device): builds the batch for a

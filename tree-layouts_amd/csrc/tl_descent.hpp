@@ -192,7 +192,13 @@ inline bool learned_fused() {
 // fastest with 4 (12.7 ms for 2^27 hits, 13.6 ms with 2), the learned descent
 // from deep start nodes, whose loads mostly miss the caches, with 2 (10.8 ms,
 // 11.8 ms with 4).
+// a caller on this host thread may fix it (the batch engine's configuration)
+inline int &btree_qpg_fixed() {
+    static thread_local int v = 0;
+    return v;
+}
 inline int btree_qpg(int def) {
+    if (btree_qpg_fixed() == 2 || btree_qpg_fixed() == 4) return btree_qpg_fixed();
     const char *e = getenv("TL_BTREE_QPG");
     if (e && (atoi(e) == 2 || atoi(e) == 4)) return atoi(e);
     return def;

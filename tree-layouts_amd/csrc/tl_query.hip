@@ -435,13 +435,20 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
 }
 
 // mvEB query shape: $TL_VEB_QPL (1, 2, 4) queries per lane and $TL_VEB_TOP
-// (0: no staged top keys) when set
+// (0: no staged top keys) when set; a caller on this host thread may fix the
+// queries per lane (the batch engine's configuration)
+int &veb_qpl_fixed() {
+    static thread_local int v = 0;
+    return v;
+}
 inline void veb_shape(int &qpl, uint32_t &topmax) {
     const char *e = getenv("TL_VEB_QPL");
     const char *f = getenv("TL_VEB_TOP");
     qpl = 2;
     topmax = VEB_TOP;
     if (e && (atoi(e) == 1 || atoi(e) == 2 || atoi(e) == 4)) qpl = atoi(e);
+    const int fx = veb_qpl_fixed();
+    if (fx == 1 || fx == 2 || fx == 4) qpl = fx;
     if (f && atoi(f) == 0) topmax = 0;
 }
 

@@ -122,7 +122,11 @@ class BatchItem(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("lds_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
-                ("wait_ms", ctypes.c_uint32)]
+                ("wait_ms", ctypes.c_uint32), ("tile_k", ctypes.c_uint32),
+                ("stream_threads", ctypes.c_uint32), ("btree_qpg", ctypes.c_uint32),
+                ("veb_qpl", ctypes.c_uint32)]
+
+BATCH_KNOBS = ("lds_bytes", "tile_k", "stream_threads", "btree_qpg", "veb_qpl")
 
 
 _OPTIONAL = {
@@ -519,7 +523,8 @@ class Batch:
     arrays: list of dicts with A (tensor), kind, layout ('bst'|'btree'|'mveb'),
     b, Q (tensor or None), ans (int64 tensor or None)."""
 
-    def __init__(self, arrays, lds_bytes=0, wg_per_cu=0, wait_ms=0):
+    def __init__(self, arrays, lds_bytes=0, wg_per_cu=0, wait_ms=0, tile_k=0, stream_threads=0,
+                 btree_qpg=0, veb_qpl=0):
         self.arrays = arrays
         items = (BatchItem * len(arrays))()
         for i, a in enumerate(arrays):
@@ -533,7 +538,7 @@ class Batch:
             items[i].nq = q.numel() if q is not None else 0
             items[i].ans = a["ans"].data_ptr() if q is not None else None
         self._items = items
-        cfg = BatchConfig(lds_bytes, wg_per_cu, wait_ms)
+        cfg = BatchConfig(lds_bytes, wg_per_cu, wait_ms, tile_k, stream_threads, btree_qpg, veb_qpl)
         h = ctypes.c_void_p()
         _check(lib().tl_batch_create(items, len(arrays), ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -563,6 +568,8 @@ class Batch:
 
 
 def batch_autotune(arrays, sample=256, stream=None):
+    """The planned path's knobs chosen by the library's autotuner on a sample
+    of the batch: a dict of BATCH_KNOBS (pass them to Batch) and sample_ms."""
     items = (BatchItem * len(arrays))()
     for i, a in enumerate(arrays):
         items[i].A = a["A"].data_ptr()
@@ -578,4 +585,6 @@ def batch_autotune(arrays, sample=256, stream=None):
     ms = ctypes.c_float()
     _check(lib().tl_batch_autotune(items, len(arrays), sample, _stream(stream), ctypes.byref(best),
                                    ctypes.byref(ms)))
-    return {"lds_bytes": best.lds_bytes, "wg_per_cu": best.wg_per_cu, "sample_ms": ms.value}
+    out = {k: getattr(best, k) for k in BATCH_KNOBS}
+    out["sample_ms"] = ms.value
+    return out
