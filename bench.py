@@ -351,7 +351,7 @@ def run_batch(args, dev):
     # tl_batch_run: every plan built and uploaded at create time, the arrays
     # that fit in LDS in batched launches, one HIP graph replayed per step
     t0 = time.perf_counter()
-    B = tl_amd.Batch(arrays, lds_bytes=args.batch_lds)
+    B = tl_amd.Batch(arrays, lds_bytes=args.batch_lds, query_queue=args.batch_query_queue)
     create_s = time.perf_counter() - t0
 
     def restore():
@@ -562,6 +562,8 @@ def main():
                     help="time the batch's permutations alone (few other dispatches)")
     ap.add_argument("--batch-lds", type=int, default=0,
                     help="LDS budget of the batch's plans in bytes (0: the library default)")
+    ap.add_argument("--batch-query-queue", type=int, default=0, choices=[0, 1],
+                    help="answer the batch's queries through one work-queue launch per layout")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: there is no multi-GPU path "

@@ -296,6 +296,10 @@ typedef struct {
                                  1024 (0 = $TL_STREAM_THREADS, else 1024) */
     uint32_t btree_qpg;       /* B-tree descents per lane group: 2 or 4 (0 = 4) */
     uint32_t veb_qpl;         /* mvEB queries per lane: 1, 2 or 4 (0 = 2) */
+    uint32_t query_queue;     /* 1: every array's queries through one persistent
+                                 work-queue launch per layout (workgroups take
+                                 chunks of 4096 queries from an atomic ticket);
+                                 0: one query launch per array */
 } tl_batch_config;
 
 /* Plans every array (the same plans as the single-array permutations) and
@@ -318,7 +322,8 @@ tl_status tl_batch_stats(void *handle, uint64_t *ops, uint64_t *tasks, uint64_t 
  * the LDS budget of the plans (32, 64, 96, 160 KiB), the bit-reversal tile
  * (32 or 64 keys a side), the threads per streamed-gather partition (256,
  * 512, 1024: the keys each thread moves), the B-tree descents per lane group
- * (2, 4) and the mvEB queries per lane (1, 2, 4).  B-tree arrays keep the b
+ * (2, 4), the mvEB queries per lane (1, 2, 4) and the query path (one launch
+ * per array, or the persistent query queues).  B-tree arrays keep the b
  * they were given: b is part of the layout the caller asked for. */
 tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t count, uint32_t sample,
                             void *stream, tl_batch_config *best, float *best_ms);

@@ -66,3 +66,24 @@ tl_status tl_permute_bst(void *, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_
 tl_status tl_permute_btree(void *, uint64_t, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_permute_btree)
 tl_status tl_permute_mveb(void *, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_permute_mveb)
 }
+
+#include "src/tl_queue.hpp"
+namespace tl {
+// the emulated persistent launch never uses the query queues
+bool queue_takes(const tl_batch_item &) { return false; }
+QueueItem queue_item(const tl_batch_item &, uint64_t, uint64_t, uint64_t, uint32_t, uint32_t,
+                     size_t *lds) {
+    *lds = 0;
+    return QueueItem{};
+}
+uint64_t queue_chunk() { return 4096; }
+tl_status queue_launch(Ctx *, uint32_t, const QueueItem *, const uint64_t *, uint32_t, uint64_t,
+                       uint64_t, unsigned long long *, size_t, hipStream_t) {
+    fprintf(stderr, "[emu] queue_launch not expected\n");
+    abort();
+}
+int &veb_qpl_fixed() {
+    static thread_local int v = 0;
+    return v;
+}
+}   // namespace tl

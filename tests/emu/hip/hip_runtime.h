@@ -18,6 +18,9 @@
 #define __shared__ static
 #define __restrict__ __restrict
 
+struct uint4 {
+    uint32_t x, y, z, w;
+};
 struct dim3 {
     uint32_t x, y, z;
     dim3(uint32_t a = 1, uint32_t b = 1, uint32_t c = 1) : x(a), y(b), z(c) {}

@@ -10,7 +10,8 @@ import numpy as np
 import oracle_lib as ol
 
 (OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
- OP_BITREV_LEVELS, OP_ROTATE) = range(9)
+ OP_BITREV_LEVELS, OP_ROTATE, OP_HEADS) = range(10)
+BITREV_PLAIN, BITREV_SHIFT, BITREV_UNFOLD, BITREV_HEAD, BITREV_ABSORB = range(5)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV, LDS_VEB = range(5)
 MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE, MV_ROW_SPREAD, MV_SPARSE_SPREAD = \
     range(6)
@@ -70,6 +71,27 @@ def rev_bits(y, D):
     return out
 
 
+def _level_slot(lev, y, K, t0):
+    """stash slot of the row end at slot y of level lev: the level's rows
+    q = (h << (lev - 2K)) | m (m the tile number) at
+    ((2^lev - 2^t0) >> K) + m * 2^K + h"""
+    q = y >> K
+    mb = lev - 2 * K
+    return (((1 << lev) - (1 << t0)) >> K) + (((q & ((1 << mb) - 1)) << K) | (q >> mb))
+
+
+def _tree_row_ends(D, K, t0):
+    """row ends (slot % 2^K = 2^K - 1) at slots >= 2^t0 of a tree of slots
+    [0, 2^D), and their stash slots"""
+    ends, slots = [], []
+    for lev in range(t0, D):
+        y = np.arange(1 << lev, dtype=np.int64)
+        e = y[(y & ((1 << K) - 1)) == (1 << K) - 1]
+        ends.append((1 << lev) + e)
+        slots.append(_level_slot(lev, e, K, t0))
+    return np.concatenate(ends), np.concatenate(slots)
+
+
 def replay(buf, A, stash=None):
     ops, blob = decode(buf)
     stash = np.zeros(0, dtype=A.dtype) if stash is None else stash
@@ -85,18 +107,62 @@ def replay(buf, A, stash=None):
         t = int(op[0])
         p = [int(v) for v in op[3:11]]
         if t == OP_LDS:
-            n, kind, b, rows, inv = p[0], p[1], p[2], p[3], p[5]
+            n, kind, b, rows, inv, shift = p[0], p[1], p[2], p[3], p[5], p[6]
             pos = lds_perm(kind, n, b, rows)
             for o in items(op, blob):
-                sub = A[o:o + n].copy()
-                if inv:
-                    A[o:o + n] = sub[pos]
-                else:
+                if inv:   # written one place right when shifted
+                    sub = A[o:o + n].copy()
+                    A[o + shift:o + shift + n] = sub[pos]
+                else:     # read from one place right when shifted
+                    sub = A[o + shift:o + shift + n].copy()
                     A[o + pos] = sub
         elif t == OP_REVERSE:
             ln = p[0]
             for o in items(op, blob):
                 A[o:o + ln] = A[o:o + ln][::-1].copy()
+        elif t == OP_BITREV and p[3] != BITREV_PLAIN:
+            # base 0 (a BST kept one place right); absorb reads the row ends
+            # at slots >= 2^t0 from the stash
+            D, K, t0 = p[1], p[2], p[4]
+            y = np.arange(1 << D, dtype=np.int64)
+            V = A[:1 << D].copy()
+            if p[3] == BITREV_ABSORB:
+                ends, slot = _tree_row_ends(D, K, t0)
+                V[ends] = stash[slot]
+            A[y] = V[rev_bits(y, D)]
+        elif t == OP_HEADS:
+            # the first 2^(t-K) slots of every level t0 .. d-1, level after level
+            t0, d, K = p[0], p[1], p[2]
+            put(0, np.concatenate([A[(1 << lev):(1 << lev) + (1 << (lev - K))]
+                                   for lev in range(t0, d)]))
+        elif t == OP_BITREV_LEVELS and p[3] == BITREV_SHIFT:
+            # every level t reversed and written one slot left: A[2^t - 1 + y]
+            # gets reversed slot y; the last slot of the level's range takes
+            # the next level's first slot from the heads (not the last level)
+            t0, d, K = p[0], p[1], p[2]
+            for lev in range(t0, d):
+                y = np.arange(1 << lev, dtype=np.int64)
+                X = A[(1 << lev) + y][rev_bits(y, lev)].copy()
+                if lev == t0:
+                    A[(1 << lev) - 1] = X[0]
+                A[(1 << lev) + y[:-1]] = X[1:]
+                if lev + 1 < d:
+                    A[(1 << (lev + 1)) - 1] = stash[((1 << (lev + 1)) - (1 << t0)) >> K]
+        elif t == OP_BITREV_LEVELS and p[3] == BITREV_UNFOLD:
+            # level t read from A[2^t - 1 ..], reversed, written to A[2^t ..];
+            # the row ends go to the stash
+            t0, d, K = p[0], p[1], p[2]
+            for lev in range(t0, d):
+                y = np.arange(1 << lev, dtype=np.int64)
+                ends = y[(y & ((1 << K) - 1)) == (1 << K) - 1]
+                slot = _level_slot(lev, ends, K, t0)
+                W = A[(1 << lev) - 1 + y][rev_bits(y, lev)].copy()
+                keep = (y & ((1 << K) - 1)) != (1 << K) - 1
+                A[(1 << lev) + y[keep]] = W[keep]
+                st = stash[:slot.max() + 1].copy() if stash.size > slot.max() else \
+                    np.concatenate([stash, np.zeros(slot.max() + 1 - stash.size, A.dtype)])
+                st[slot] = W[ends]
+                put(0, st)
         elif t == OP_BITREV:
             base, D = np.int64(p[0]), p[1]
             y = np.arange(1 << D, dtype=np.int64)

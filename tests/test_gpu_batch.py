@@ -145,6 +145,7 @@ def test_batch_planned_path_autotune(dev):
     assert cfg["lds_bytes"] in (0, 32768, 65536, 98304, 163840)
     assert cfg["tile_k"] in (5, 6) and cfg["stream_threads"] in (256, 512, 1024)
     assert cfg["btree_qpg"] in (2, 4) and cfg["veb_qpl"] in (1, 2, 4)
+    assert cfg["query_queue"] in (0, 1)
     assert 0 < cfg["sample_ms"] < 1e29
     B = tl_amd.Batch(arrays, **{k: cfg[k] for k in tl_amd.BATCH_KNOBS})
     B.run(graph=False)
@@ -161,7 +162,7 @@ def test_batch_planned_path_autotune(dev):
 @pytest.mark.parametrize("knobs", [{"tile_k": 6, "stream_threads": 256, "btree_qpg": 2,
                                     "veb_qpl": 1},
                                    {"tile_k": 5, "stream_threads": 512, "btree_qpg": 4,
-                                    "veb_qpl": 4, "lds_bytes": 32768}])
+                                    "veb_qpl": 4, "lds_bytes": 32768, "query_queue": 1}])
 def test_batch_knobs_keep_results(knobs, dev):
     """Every knob the autotuner turns leaves layouts and answers unchanged."""
     arrays, host = make_batch(120, 4, 18, 13, dev)
@@ -171,4 +172,47 @@ def test_batch_knobs_keep_results(knobs, dev):
     check(arrays, host)
     B.close()
 
+
+@pytest.mark.parametrize("lds,chunk", [(0, None), (32768, None), (0, "1024"), (0, "65536")])
+def test_batch_query_queue_matches_oracle(lds, chunk, dev, monkeypatch):
+    """The persistent query queues (one work-queue launch per layout:
+    workgroups take chunks of queries of any array from an atomic ticket)
+    answer every array like the oracle, eagerly and on three graph replays;
+    chunks of 1024 to 65536 queries"""
+    if chunk:
+        monkeypatch.setenv("TL_QQ_CHUNK", chunk)
+    arrays, host = make_batch(240, 0, 18, 5, dev)
+    B = tl_amd.Batch(arrays, lds_bytes=lds, query_queue=1)
+    B.run(graph=False)
+    torch.cuda.synchronize()
+    check(arrays, host)
+    for _ in range(3):
+        restore(arrays, host, dev)
+        torch.cuda.synchronize()
+        B.run(graph=True)
+        torch.cuda.synchronize()
+        check(arrays, host)
+    B.close()
+
+
+def test_batch_query_queue_with_leftovers(dev):
+    """B-tree arrays whose nodes are not whole 16-byte words (b = 3) are
+    answered by their own launches beside the queues; all answers match"""
+    arrays, host = make_batch(90, 4, 17, 23, dev)
+    bs = []
+    for i, a in enumerate(arrays):
+        b = 3 if (a["layout"] == "btree" and i % 2) else 8
+        a["b"] = b
+        bs.append(b)
+    B = tl_amd.Batch(arrays, query_queue=1)
+    B.run(graph=True)
+    torch.cuda.synchronize()
+    for a, (keys, q), b in zip(arrays, host, bs):
+        exp_layout = ol.permute(a["layout"], keys.copy(), b)
+        got = a["A"].cpu().numpy().view(keys.dtype)
+        assert np.array_equal(got, exp_layout), (a["layout"], b, keys.size)
+        exp_ans = ol.search(a["layout"], exp_layout, q, a["kind"], b)
+        assert np.array_equal(a["ans"].cpu().numpy().view(np.uint64), exp_ans), \
+            (a["layout"], b, keys.size)
+    B.close()
 

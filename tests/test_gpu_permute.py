@@ -77,7 +77,46 @@ def test_bst_bitrev_tile_sizes(tile_k, kind, dev, monkeypatch):
         assert np.array_equal(A.cpu().numpy().astype(np.int64), np.arange(n)), (tile_k, kind, n)
 
 
+@pytest.mark.parametrize("grid", ["8", "32"])
+@pytest.mark.parametrize("tile_k", ["5", "6"])
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+def test_bst_bitrev_looped_grid(grid, tile_k, kind, dev, monkeypatch):
+    """Bit-reversal tiles taken in a loop by a grid of fewer workgroups give
+    the same BST layout (both phases), and the inverse restores sorted order"""
+    monkeypatch.setenv("TL_BITREV_GRID", grid)
+    monkeypatch.setenv("TL_BITREV_K", tile_k)
+    for n in [4095, 65535, 300007, 1048575, 1048576, 3_000_017, (1 << 23) - 1, 1 << 23]:
+        got, exp = _run("bst", 1, kind, n, dev)
+        assert np.array_equal(got, exp), (grid, tile_k, kind, n)
+        A = torch.from_numpy(got.astype(np.int32 if kind == "u32" else np.int64)).to(dev)
+        tl_amd.unpermute("bst", A, kind=kind)
+        torch.cuda.synchronize()
+        assert np.array_equal(A.cpu().numpy().astype(np.int64), np.arange(n)), (grid, kind, n)
+
+
+@pytest.mark.parametrize("fold", ["0", "1"])
+@pytest.mark.parametrize("tile_k", ["5", "6"])
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+@pytest.mark.parametrize("xcd", ["0", "1"])
+def test_bst_leaf_fold(fold, tile_k, kind, xcd, dev, monkeypatch):
+    """Trees of 2^h keys: the one leaf's move folded into phase one (row ends
+    through the stash) and the unfolded plan (leaf gather, then the tiles) both
+    give the oracle's layout, and the inverse restores sorted order; tiles
+    numbered in workgroup order or XCD by XCD"""
+    monkeypatch.setenv("TL_BST_FOLD", fold)
+    monkeypatch.setenv("TL_BITREV_K", tile_k)
+    monkeypatch.setenv("TL_BITREV_XCD", xcd)
+    for n in [1 << 15, 1 << 16, 1 << 17, 1 << 20, 1 << 23, (1 << 23) - 1, 3_000_017]:
+        got, exp = _run("bst", 1, kind, n, dev)
+        assert np.array_equal(got, exp), (fold, tile_k, kind, n)
+        A = torch.from_numpy(got.astype(np.int32 if kind == "u32" else np.int64)).to(dev)
+        tl_amd.unpermute("bst", A, kind=kind)
+        torch.cuda.synchronize()
+        assert np.array_equal(A.cpu().numpy().astype(np.int64), np.arange(n)), (fold, kind, n)
+
+
 @pytest.mark.parametrize("layout,b,n", [("bst", 1, (1 << 24) - 1), ("bst", 1, (1 << 24) + 12345),
+                                        ("bst", 1, 1 << 24), ("bst", 1, 1 << 26),
                                         ("btree", 16, 17 ** 6 - 1), ("btree", 16, 30_000_001),
                                         ("btree", 8, 9 ** 7 + 5), ("mveb", 0, (1 << 24) - 1),
                                         ("mveb", 0, 30_000_001), ("veb", 0, (1 << 23) - 1),

@@ -48,8 +48,8 @@ def test_plans_replay_to_oracle(layout, bs, lds, row_stash, monkeypatch):
     one plan)"""
     if row_stash is not None:
         monkeypatch.setenv("TL_ROW_STASH_MAX", row_stash)
-    sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 5000, 17 ** 3 - 1, 17 ** 3,
-             33333, 65535, 65536]
+    sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 4096, 5000, 8192,
+             17 ** 3 - 1, 17 ** 3, 33333, 65535, 65536]
     for b in bs:
         for n in sizes:
             buf = tl_amd.plan_export(layout, n, b, "u64", lds)
@@ -79,8 +79,8 @@ def test_inverse_plans_replay_to_identity(layout, bs, lds, row_stash, monkeypatc
     if row_stash is not None:
         monkeypatch.setenv("TL_ROW_STASH_MAX", row_stash)
     rng = np.random.default_rng(31)
-    sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 5000, 17 ** 3 - 1, 17 ** 3,
-             33333, 65535, 65536]
+    sizes = [1, 2, 3, 7, 16, 17, 100, 255, 256, 1000, 1023, 4095, 4096, 5000, 8192,
+             17 ** 3 - 1, 17 ** 3, 33333, 65535, 65536]
     for b in bs:
         for n in sizes:
             inv = tl_amd.plan_export(layout, n, b, "u64", lds, inverse=True)

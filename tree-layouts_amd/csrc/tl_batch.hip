@@ -26,6 +26,7 @@
 #include "tl_internal.hpp"
 #include "tl_kernels.hpp"
 #include "tl_plan.hpp"
+#include "tl_queue.hpp"
 
 namespace tl {
 
@@ -521,7 +522,7 @@ struct WaveStep {
 };
 struct QTables {                  // device mvEB table set of one array
     uint64_t tables_at = 0, idx_at = 0, tidx_at = 0;
-    uint32_t nt = 0;
+    uint32_t nt = 0, tsize = 0;       // structures, table entries
 };
 
 struct Batch {
@@ -546,6 +547,15 @@ struct Batch {
     hipEvent_t ev_fork[2] = {}, ev_join[2][NSUB] = {};
     size_t stash_stride = 0, flag_stride = 0;
     std::vector<int> large_branch, query_branch, small_branch;
+    // query queues (one persistent launch per layout): records and chunk
+    // prefix sums in d_queue, tickets in the flag region (cleared every run)
+    std::vector<uint8_t> queued;   // per item: answered by a queue
+    uint64_t q_items_at[3] = {0, 0, 0}, q_pre_at[3] = {0, 0, 0}, q_chunks[3] = {0, 0, 0};
+    uint32_t q_count[3] = {0, 0, 0};
+    size_t q_lds[3] = {0, 0, 0};
+    size_t q_ticket_at = 0;
+    uint64_t q_chunk = 4096;       // queries per queue ticket
+    void *d_queue = nullptr;
     tl_batch_config cfg{};
     std::vector<BOp> ops;
     std::vector<BTask> tasks;
@@ -568,7 +578,7 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : {(void *)d_ops, (void *)d_tasks, (void *)d_blob, (void *)d_done,
                         (void *)d_ticket, (void *)d_flags, (void *)d_err, d_stash, d_pdata,
-                        d_pstash, (void *)d_pflags, d_wdata, d_wstash, (void *)d_wctr})
+                        d_pstash, (void *)d_pflags, d_wdata, d_wstash, (void *)d_wctr, d_queue})
             if (p) (void)hipFree(p);
         for (int k = 0; k < NSUB; k++)
             if (sub[k]) (void)hipStreamDestroy(sub[k]);
@@ -647,8 +657,10 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     Tuning tu_persistent;
     tu_persistent.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
     // the persistent executors rotate by reversals (they have no lane walk)
+    // and have no folded bit reversal
     tu_persistent.rotate_walk = false;
     tu_persistent.fuse_leaves = false;
+    tu_persistent.bst_fold = false;
     struct Pending {
         uint32_t op, depth, array;
     };
@@ -800,8 +812,54 @@ static tl_status upload_batch(Batch &B) {
     return TL_OK;
 }
 
+// the query queues: every array the queue kernels take, grouped by layout,
+// with its chunks of q_chunk queries counted by prefix sums; the mvEB table
+// sets are the ones already uploaded in d_pdata
+static tl_status build_queues(Batch &B) {
+    const uint64_t base = (uint64_t)B.d_pdata;
+    std::vector<QueueItem> recs[3];
+    std::vector<uint64_t> pre[3];
+    B.queued.assign(B.items.size(), 0);
+    B.q_chunk = queue_chunk();
+    for (uint32_t a = 0; a < B.items.size(); a++) {
+        const tl_batch_item &it = B.items[a];
+        if (!queue_takes(it)) continue;
+        B.queued[a] = 1;
+        const QTables &q = B.qt[a];
+        size_t lds = 0;
+        const QueueItem r = queue_item(it, base + q.tables_at, base + q.idx_at, base + q.tidx_at,
+                                       q.nt, q.tsize, &lds);
+        const uint32_t L = it.layout;
+        pre[L].push_back(B.q_chunks[L]);
+        B.q_chunks[L] += (it.nq + B.q_chunk - 1) / B.q_chunk;
+        recs[L].push_back(r);
+        B.q_lds[L] = std::max(B.q_lds[L], lds);
+    }
+    size_t bytes = 0;
+    for (int L = 0; L < 3; L++) {
+        B.q_count[L] = (uint32_t)recs[L].size();
+        B.q_items_at[L] = bytes;
+        bytes += (recs[L].size() * sizeof(QueueItem) + 255) & ~(size_t)255;
+        B.q_pre_at[L] = bytes;
+        bytes += (pre[L].size() * 8 + 255) & ~(size_t)255;
+    }
+    if (!bytes) return TL_OK;
+    TL_HIP(hipMalloc(&B.d_queue, bytes));
+    for (int L = 0; L < 3; L++) {
+        unsigned char *d = (unsigned char *)B.d_queue;
+        if (!recs[L].empty()) {
+            TL_HIP(hipMemcpy(d + B.q_items_at[L], recs[L].data(),
+                             recs[L].size() * sizeof(QueueItem), hipMemcpyHostToDevice));
+            TL_HIP(hipMemcpy(d + B.q_pre_at[L], pre[L].data(), pre[L].size() * 8,
+                             hipMemcpyHostToDevice));
+        }
+    }
+    return TL_OK;
+}
+
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
+    tu.bst_fold = false;   // the multi-array bit-reversal launch has no folded form
     if (B.cfg.tile_k == 5 || B.cfg.tile_k == 6) tu.bitrev_k = B.cfg.tile_k;
     if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;
@@ -870,6 +928,7 @@ static tl_status build_planned(Batch &B, Ctx *c) {
             q.idx_at = put(ix.data(), ix.size() * 8);
             q.tidx_at = put(t32.data(), t32.size() * 4);
             q.nt = (uint32_t)ix.size();
+            q.tsize = (uint32_t)t.size();
         }
     }
     for (SmallGroup &g : B.small) g.at = put(g.desc.data(), g.desc.size() * 8);
@@ -877,9 +936,12 @@ static tl_status build_planned(Batch &B, Ctx *c) {
     if (!data.empty())
         TL_HIP(hipMemcpy(B.d_pdata, data.data(), data.size(), hipMemcpyHostToDevice));
     B.stash_stride = (stash_bytes + 255) & ~255ull;
+    B.q_ticket_at = flag_bytes;   // the query queues' three tickets
+    flag_bytes += 256;
     B.flag_stride = flag_bytes;   // one region for all items: cleared once per run
     TL_HIP(hipMalloc(&B.d_pstash, B.stash_stride * Batch::NSUB));
     TL_HIP(hipMalloc((void **)&B.d_pflags, flag_bytes));
+    TL_TRY(build_queues(B));
     for (int k = 0; k < Batch::NSUB; k++)
         TL_HIP(hipStreamCreateWithFlags(&B.sub[k], hipStreamNonBlocking));
     for (int f = 0; f < 2; f++) {
@@ -1218,9 +1280,24 @@ struct QueryShape {
 static tl_status run_queries(Batch &B, const unsigned char *base, hipStream_t s) {
     QueryShape shape(B.cfg);
     TL_TRY(fork_branches(B, 1, s));
+    const bool queues = B.cfg.query_queue && B.d_queue;
+    if (queues) {
+        // one persistent launch per layout, each on a branch of its own; the
+        // tickets were cleared with the flag region at the start of the run
+        const unsigned char *d = (const unsigned char *)B.d_queue;
+        unsigned long long *tickets =
+            (unsigned long long *)((unsigned char *)B.d_pflags + B.q_ticket_at);
+        Ctx *c;
+        TL_TRY(get_ctx(&c));
+        for (uint32_t L = 0; L < 3; L++)
+            TL_TRY(queue_launch(c, L, (const QueueItem *)(d + B.q_items_at[L]),
+                                (const uint64_t *)(d + B.q_pre_at[L]), B.q_count[L],
+                                B.q_chunks[L], B.q_chunk, tickets + L, B.q_lds[L],
+                                B.sub[L]));
+    }
     for (uint32_t a = 0; a < B.items.size(); a++) {
         const tl_batch_item &it = B.items[a];
-        if (!it.nq) continue;
+        if (!it.nq || (queues && B.queued[a])) continue;
         hipStream_t q_s = B.sub[B.query_branch[a]];
         if (it.layout == 0) {
             TL_TRY(tl_query_bst(it.A, it.n, it.dtype, it.Q, it.nq, it.ans, q_s, nullptr));
@@ -1473,6 +1550,7 @@ extern "C" tl_status tl_batch_autotune(const tl_batch_item *items, uint32_t coun
         {&tl_batch_config::stream_threads, {256u, 512u, 1024u}},
         {&tl_batch_config::btree_qpg, {2u, 4u}},
         {&tl_batch_config::veb_qpl, {1u, 2u, 4u}},
+        {&tl_batch_config::query_queue, {0u, 1u}},
     };
     for (const auto &kv : knobs) {
         for (uint32_t v : kv.second) {

@@ -38,9 +38,34 @@ __device__ inline uint64_t bitrev_tile_order(uint64_t t, uint32_t MB, uint32_t p
     return (Ahi << (sb + c)) | (C << sb) | B;
 }
 
-// one tile pair of a D-bit reversal at base: tile number t of 2^(D-2K)
-template <typename T, int K>
-__device__ inline void bitrev_tile(T *__restrict__ A, int64_t base, uint32_t D, uint64_t t) {
+// stash slot of the row end at slot Y >= 2^t0 (Y % 2^K = 2^K - 1) of a BST
+// stored one place right (slot Y at A[Y]), t0 = 2K: level t = floor(log2 Y)
+// holds its rows q = (h << (t - 2K)) | m (m its tile number) at
+// ((2^t - 2^t0) >> K) + m * 2^K + h, so that each tile's row ends are
+// contiguous
+template <int K>
+__device__ inline uint64_t rowend_slot(uint64_t Y, uint32_t t0) {
+    const uint32_t t = 63u - (uint32_t)__clzll(Y);
+    const uint32_t MBt = t - 2 * K;
+    const uint64_t q = (Y - (1ull << t)) >> K;
+    const uint64_t h = q >> MBt, m = q & ((1ull << MBt) - 1);
+    return (((1ull << t) - (1ull << t0)) >> K) + ((m << K) | h);
+}
+
+// one tile pair of a D-bit reversal at base: tile number t of 2^(D-2K).
+// The forms (FOLD), with slot y of the tile and l its place in its row:
+//   0: slot y is A[base + y] (slot -1 does not exist) and stays there;
+//   2 (unfold; a level of a BST whose final places are one left of where it
+//      is kept): read A[base + y], write A[base + 1 + y], or, when
+//      l = 2^K - 1, S[m * 2^K + h] (the tile's row ends contiguous);
+//   4 (absorb, base 0): as 0, with the row ends at slots >= 2^t0 read from
+//      S[rowend_slot(y)], where unfolding the levels left them.
+// A tile pair reads all its slots before it writes any, and every key a tile
+// writes over is one of its own slots' or a row end put in S, so tiles need
+// no order among themselves.
+template <typename T, int K, int FOLD = 0>
+__device__ inline void bitrev_tile(T *__restrict__ A, int64_t base, uint32_t D, uint64_t t,
+                                   T *__restrict__ S = nullptr, uint32_t t0 = 0) {
     constexpr int R = 1 << K;
     constexpr int E = R * R / 256;   // elements per thread and tile
     __shared__ T sA[R][R + 1];
@@ -51,51 +76,205 @@ __device__ inline void bitrev_tile(T *__restrict__ A, int64_t base, uint32_t D, 
     const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
     if (m > mr) return;
     const uint32_t hs = D - K;
+    auto load = [&](uint64_t mm, uint32_t h, uint32_t l) -> T {
+        const uint64_t y = ((uint64_t)h << hs) | (mm << K) | l;
+        if constexpr (FOLD == 4) {
+            return (l == R - 1 && (y >> t0) != 0) ? S[rowend_slot<K>(y, t0)] : A[y];
+        } else {
+            return A[base + (int64_t)y];
+        }
+    };
+    auto store = [&](uint64_t mm, uint32_t h, uint32_t l, T v) {
+        const uint64_t y = ((uint64_t)h << hs) | (mm << K) | l;
+        if constexpr (FOLD == 2) {
+            if (l == R - 1)
+                S[(mm << K) | h] = v;
+            else
+                A[base + 1 + (int64_t)y] = v;
+        } else {
+            A[base + (int64_t)y] = v;
+        }
+    };
+    // slot 0 of tile 0 does not exist when base = -1
+    const bool slot0 = base >= 0;
 #pragma unroll
     for (int k = 0; k < E; k++) {
         const uint32_t e = threadIdx.x + k * 256;
         const uint32_t h = e >> K, l = e & (R - 1);
-        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
-        if (ia >= 0) sA[h][l] = A[ia];
-        if (m != mr) {
-            const int64_t ib = base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l);
-            sB[h][l] = A[ib];
-        }
+        if (m != 0 || e != 0 || slot0) sA[h][l] = load(m, h, l);
+        if (m != mr) sB[h][l] = load(mr, h, l);
     }
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
         const uint32_t h = e >> K, l = e & (R - 1);
         const uint32_t hr = rev_low_bits32(l, K), lr = rev_low_bits32(h, K);
-        const int64_t ia = base + (int64_t)(((uint64_t)h << hs) | (m << K) | l);
         if (m != mr) {
-            const int64_t ib = base + (int64_t)(((uint64_t)h << hs) | (mr << K) | l);
-            A[ia] = sB[hr][lr];
-            A[ib] = sA[hr][lr];
-        } else if (ia >= 0) {
-            A[ia] = sA[hr][lr];
+            store(m, h, l, sB[hr][lr]);
+            store(mr, h, l, sA[hr][lr]);
+        } else if (m != 0 || e != 0 || slot0) {
+            store(m, h, l, sA[hr][lr]);
         }
     }
 }
 
+// one tile pair of the reversal of one level of a BST kept one place right of
+// its final places (slot y of the level at A[base + 1 + y], D its depth),
+// written in place one slot left: the kept row (h, m) gets the values of its
+// slots 1 .. 2^K - 1 and, last, of the first slot of the next row.  That value
+// comes from the heads H: the level's first 2^(D-K) slots (row 0 of every
+// tile) as they were before the launch, then the next level's first slot.
+// All reads and writes are whole aligned rows of the tile pair's own.
+// first: tile 0 also writes A[base], the final place of the level's first
+// slot; last: the level's last kept slot is not written (the tree's leaf goes
+// there).
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
-                                                      uint32_t D) {
+__device__ inline void bitrev_tile_shift(T *__restrict__ A, int64_t base, uint32_t D, uint64_t t,
+                                         const T *__restrict__ H, bool first, bool last) {
+    constexpr int R = 1 << K;
+    constexpr int E = R * R / 256;
+    __shared__ T sA[R][R + 1];
+    __shared__ T sB[R][R + 1];
+    __shared__ T nA[R], nB[R];
     const uint32_t MB = D - 2 * K;
-    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    constexpr uint32_t row_log2 = (uint32_t)K + (sizeof(T) == 8 ? 3u : 2u);
+    const uint64_t m = bitrev_tile_order(t, MB, 21u - row_log2);
+    const uint64_t mr = MB ? (__brevll(m) >> (64 - MB)) : 0;
+    if (m > mr) return;
+    const uint32_t hs = D - K;
+    T *Y = A + base + 1;   // slot y of the level at Y[y]
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const uint32_t e = threadIdx.x + k * 256;
+        const uint32_t h = e >> K, l = e & (R - 1);
+        sA[h][l] = Y[((uint64_t)h << hs) | (m << K) | l];
+        if (m != mr) sB[h][l] = Y[((uint64_t)h << hs) | (mr << K) | l];
+    }
+    // first slot of the row after (h, mm): row (h, mm + 1), or (h + 1, 0), or
+    // the next level's first slot; after the reversal slot (h', m', 0) holds
+    // what slot (0, rev m', rev h') held, which is H[rev m' * 2^K + rev h']
+    if (threadIdx.x < 2 * R) {
+        const uint32_t h = threadIdx.x & (R - 1);
+        const uint64_t mm = threadIdx.x < R ? m : mr;
+        T v{};
+        if (mm + 1 < (1ull << MB)) {
+            const uint64_t mn = MB ? (__brevll(mm + 1) >> (64 - MB)) : 0;
+            v = H[(mn << K) | rev_low_bits32(h, K)];
+        } else if (h + 1 < (uint32_t)R) {
+            v = H[rev_low_bits32(h + 1, K)];
+        } else if (!last) {
+            v = H[1ull << (D - K)];
+        }
+        if (threadIdx.x < R)
+            nA[h] = v;
+        else
+            nB[h] = v;
+    }
+    __syncthreads();
+    const uint64_t lastrow = (1ull << MB) - 1;
+    for (uint32_t e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const uint32_t h = e >> K, j = e & (R - 1);
+        const bool end = j == R - 1;
+        const uint32_t hr = end ? 0u : rev_low_bits32(j + 1, K), lr = rev_low_bits32(h, K);
+        const bool skip = last && end && h == R - 1;   // the level's last kept slot
+        if (m != mr) {
+            if (!(skip && m == lastrow))
+                Y[((uint64_t)h << hs) | (m << K) | j] = end ? nA[h] : sB[hr][lr];
+            if (!(skip && mr == lastrow))
+                Y[((uint64_t)h << hs) | (mr << K) | j] = end ? nB[h] : sA[hr][lr];
+        } else if (!(skip && m == lastrow)) {
+            Y[((uint64_t)h << hs) | (m << K) | j] = end ? nA[h] : sA[hr][lr];
+        }
+    }
+    if (first && m == 0 && threadIdx.x == 0) A[base] = sA[0][0];
+}
+
+// tile number of workgroup b when `xcd` is set: workgroups go round robin to
+// the 8 XCDs (b % 8), each with its own L2 and address translation cache, so
+// XCD x takes the x-th eighth of the tile numbers in order and neighbouring
+// tiles (which share pages and the cache lines at their row ends) run on one
+// XCD.  The launch makes its workgroup count a multiple of 8.
+__device__ inline uint64_t tile_of(uint32_t xcd) {
+    const uint64_t b = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (!xcd) return b;
+    const uint64_t per = ((uint64_t)gridDim.x * gridDim.y) >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename T, int K, int FOLD = 0>
+__global__ __launch_bounds__(256) void k_bitrev_tiles(T *__restrict__ A, int64_t base,
+                                                      uint32_t D, T *__restrict__ S,
+                                                      uint32_t t0, uint32_t xcd) {
+    const uint32_t MB = D - 2 * K;
+    const uint64_t t = tile_of(xcd);
     if (MB < 64 && (t >> MB) != 0) return;
-    bitrev_tile<T, K>(A, base, D, t);
+    bitrev_tile<T, K, FOLD>(A, base, D, t, S, t0);
+}
+
+// the same tiles with a grid of fewer workgroups, each taking tile numbers
+// t = blockIdx.x, + gridDim.x, ...: the half of the numbers whose pair
+// belongs to the partner tile cost a loop trip instead of a workgroup
+template <typename T, int K, int FOLD = 0>
+__global__ __launch_bounds__(256) void k_bitrev_tiles_loop(T *__restrict__ A, int64_t base,
+                                                           uint32_t D, uint64_t tiles,
+                                                           T *__restrict__ S, uint32_t t0) {
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        bitrev_tile<T, K, FOLD>(A, base, D, t, S, t0);
+        __syncthreads();   // the tile buffers are reused by the next trip
+    }
 }
 
 // the per-level reversals of levels t0 .. d-1 (t0 = 2K) in one launch: level t
 // holds 2^t keys at base 2^t - 1 and 2^(t-2K) tiles; tile number g of the
-// launch is tile g - (2^j - 1) of level t0 + j, j = floor(log2(g + 1))
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_bitrev_levels(T *__restrict__ A, uint32_t t0, uint32_t d) {
-    const uint64_t g = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
-    if (g >= (1ull << (d - t0)) - 1) return;
+// launch is tile g - (2^j - 1) of level t0 + j, j = floor(log2(g + 1)).
+// FOLD 1 (shift) / 2 (unfold): the tree kept one place right; level t's part
+// of S (heads or row ends) starts at (2^t - 2^t0) >> K
+template <typename T, int K, int FOLD>
+__device__ inline void level_tile(T *__restrict__ A, uint32_t t0, uint32_t d, uint64_t g,
+                                  T *__restrict__ S) {
     const uint32_t j = 63u - (uint32_t)__clzll(g + 1);
     const uint32_t t = t0 + j;
-    bitrev_tile<T, K>(A, (int64_t)((1ull << t) - 1), t, g - ((1ull << j) - 1));
+    const int64_t base = (int64_t)((1ull << t) - 1);
+    const uint64_t tile = g - ((1ull << j) - 1);
+    if constexpr (FOLD == 1) {
+        bitrev_tile_shift<T, K>(A, base, t, tile, S + (((1ull << t) - (1ull << t0)) >> K),
+                                t == t0, t + 1 == d);
+    } else if constexpr (FOLD == 2) {
+        bitrev_tile<T, K, 2>(A, base, t, tile, S + (((1ull << t) - (1ull << t0)) >> K));
+    } else {
+        bitrev_tile<T, K>(A, base, t, tile);
+    }
+}
+
+template <typename T, int K, int FOLD = 0>
+__global__ __launch_bounds__(256) void k_bitrev_levels(T *__restrict__ A, uint32_t t0, uint32_t d,
+                                                       uint32_t xcd, T *__restrict__ S) {
+    const uint64_t g = tile_of(xcd);
+    if (g >= (1ull << (d - t0)) - 1) return;
+    level_tile<T, K, FOLD>(A, t0, d, g, S);
+}
+
+template <typename T, int K, int FOLD = 0>
+__global__ __launch_bounds__(256) void k_bitrev_levels_loop(T *__restrict__ A, uint32_t t0,
+                                                            uint32_t d, T *__restrict__ S) {
+    const uint64_t total = (1ull << (d - t0)) - 1;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        level_tile<T, K, FOLD>(A, t0, d, g, S);
+        __syncthreads();   // the tile buffers are reused by the next trip
+    }
+}
+
+// the heads of levels t0 .. d-1 of a BST kept one place right (level t's
+// first 2^(t-K) slots, at A[2^t ..]) to S, level after level: what the
+// shifted level reversal reads for the first slots of the rows after its own
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_level_heads(const T *__restrict__ A, T *__restrict__ S,
+                                                     uint32_t t0, uint32_t d) {
+    const uint64_t total = ((1ull << d) - (1ull << t0)) >> K;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t = 63u - (uint32_t)__clzll((i << K) + (1ull << t0));
+        S[i] = A[(1ull << t) + i - (((1ull << t) - (1ull << t0)) >> K)];
+    }
 }
 
 // ---------------------------------------------------------------------------

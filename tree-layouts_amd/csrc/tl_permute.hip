@@ -54,12 +54,15 @@ __device__ inline uint32_t lds_slot(uint32_t x, uint32_t n, uint32_t kind, uint3
 
 // inv = 1 applies the inverse permutation (slot -> rank instead of rank ->
 // slot): the keys go into LDS in place and come out gathered through the slots.
+// shift = 1: the item's keys are read from one place right (inverse: written
+// one place right).
 // one workgroup's share: items [group*per, group*per + per) of the runs
 template <typename T>
 __device__ inline void lds_body(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                 uint32_t nruns, uint64_t nitems, uint32_t per, uint32_t n,
                                 uint32_t kind, uint32_t b, uint32_t rows, uint32_t magic,
-                                uint32_t inv, uint64_t group) {
+                                uint32_t inv, uint64_t group, uint32_t shift = 0) {
+    const uint32_t rshift = inv ? 0u : shift, wshift = inv ? shift : 0u;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *s = reinterpret_cast<T *>(smem_raw);
     uint32_t *tbl = reinterpret_cast<uint32_t *>(smem_raw + (size_t)per * n * sizeof(T));
@@ -84,7 +87,7 @@ __device__ inline void lds_body(T *__restrict__ A, const uint64_t *__restrict__ 
             if (i < tot) {
                 it[u] = table ? __umulhi(i, magic) : 0;
                 ii[u] = i - it[u] * n;
-                v[u] = A[s_off[it[u]] + ii[u]];
+                v[u] = A[s_off[it[u]] + ii[u] + rshift];
             }
         }
 #pragma unroll
@@ -101,7 +104,7 @@ __device__ inline void lds_body(T *__restrict__ A, const uint64_t *__restrict__ 
     for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
         const uint32_t t = table ? __umulhi(i, magic) : 0, x = i - t * n;
         const uint32_t p = inv ? (table ? tbl[x] : lds_slot(x, n, kind, b, rows, sh)) : x;
-        A[s_off[t] + x] = s[t * n + p];
+        A[s_off[t] + x + wshift] = s[t * n + p];
     }
 }
 
@@ -109,8 +112,9 @@ template <typename T>
 __global__ __launch_bounds__(512) void k_lds_runs(T *__restrict__ A, const uint64_t *__restrict__ runs,
                                                   uint32_t nruns, uint64_t nitems, uint32_t per,
                                                   uint32_t n, uint32_t kind, uint32_t b,
-                                                  uint32_t rows, uint32_t magic, uint32_t inv) {
-    lds_body<T>(A, runs, nruns, nitems, per, n, kind, b, rows, magic, inv, blockIdx.x);
+                                                  uint32_t rows, uint32_t magic, uint32_t inv,
+                                                  uint32_t shift) {
+    lds_body<T>(A, runs, nruns, nitems, per, n, kind, b, rows, magic, inv, blockIdx.x, shift);
 }
 
 // reverse every segment [off, off+len): REV_U mirrored pairs per thread per
@@ -408,6 +412,96 @@ __global__ __launch_bounds__(256) void k_sparse_stream_fused(T *__restrict__ A,
 // ---------------------------------------------------------------------------
 // executor
 
+// steps of a rotation lane walk with their loads in flight together:
+// $TL_WALK_DEPTH (4 or 8), else 4
+static int walk_depth() {
+    const char *e = getenv("TL_WALK_DEPTH");
+    return (e && atoi(e) == 8) ? 8 : 4;
+}
+
+// workgroups per compute unit of a chained-move launch (moves are taken
+// from a ticket, so more workgroups only add moves in flight):
+// $TL_CHAIN_WG_PER_CU (1..64), else 8
+static uint64_t chained_wg_per_cu() {
+    const char *e = getenv("TL_CHAIN_WG_PER_CU");
+    const int v = e ? atoi(e) : 8;
+    return (v >= 1 && v <= 64) ? (uint64_t)v : 8u;
+}
+
+// bit-reversal tiles: one workgroup per tile number (0, the default), or
+// $TL_BITREV_GRID workgroups per compute unit (1..64) looping over them
+static uint64_t bitrev_grid_per_cu() {
+    const char *e = getenv("TL_BITREV_GRID");
+    const int v = e ? atoi(e) : 0;
+    return (v >= 1 && v <= 64) ? (uint64_t)v : 0u;
+}
+
+// bit-reversal tiles numbered XCD by XCD ($TL_BITREV_XCD=1), else in
+// workgroup order
+static uint32_t bitrev_xcd() {
+    const char *e = getenv("TL_BITREV_XCD");
+    return (e && atoi(e) == 1) ? 1u : 0u;
+}
+
+// a D-bit reversal at base (tile numbers < tiles): one workgroup per tile
+// number, or `per` workgroups per compute unit looping over them
+template <typename T, int FOLD>
+static tl_status bitrev_launch(Ctx *c, T *A, int64_t base, uint32_t D, uint32_t K, uint64_t tiles,
+                               uint32_t per, T *S, uint32_t t0, hipStream_t s) {
+    if (per) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)c->num_cus * per);
+        if (K == 6)
+            hipLaunchKernelGGL((k_bitrev_tiles_loop<T, 6, FOLD>), dim3(g), dim3(256), 0, s, A,
+                               base, D, tiles, S, t0);
+        else
+            hipLaunchKernelGGL((k_bitrev_tiles_loop<T, 5, FOLD>), dim3(g), dim3(256), 0, s, A,
+                               base, D, tiles, S, t0);
+    } else {
+        const uint32_t xcd = bitrev_xcd();
+        const uint64_t padded = xcd ? (tiles + 7) & ~7ull : tiles;   // tiles: a power of 2
+        const uint32_t gx = (uint32_t)std::min<uint64_t>(padded, 1u << 20);
+        const uint32_t gy = (uint32_t)((padded + gx - 1) / gx);
+        if (K == 6)
+            hipLaunchKernelGGL((k_bitrev_tiles<T, 6, FOLD>), dim3(gx, gy), dim3(256), 0, s, A,
+                               base, D, S, t0, xcd);
+        else
+            hipLaunchKernelGGL((k_bitrev_tiles<T, 5, FOLD>), dim3(gx, gy), dim3(256), 0, s, A,
+                               base, D, S, t0, xcd);
+    }
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
+// the per-level reversals of levels t0 .. d-1 in one launch
+template <typename T, int FOLD>
+static tl_status levels_launch(Ctx *c, T *A, uint32_t t0, uint32_t d, uint32_t K, uint32_t per,
+                               T *S, hipStream_t s) {
+    const uint64_t tiles = (1ull << (d - t0)) - 1;
+    if (per) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)c->num_cus * per);
+        if (K == 6)
+            hipLaunchKernelGGL((k_bitrev_levels_loop<T, 6, FOLD>), dim3(g), dim3(256), 0, s, A,
+                               t0, d, S);
+        else
+            hipLaunchKernelGGL((k_bitrev_levels_loop<T, 5, FOLD>), dim3(g), dim3(256), 0, s, A,
+                               t0, d, S);
+    } else {
+        // tiles + 1 is a power of 2: padded, the grid is gx * gy exactly
+        const uint32_t xcd = bitrev_xcd();
+        const uint64_t padded = xcd ? (tiles + 8) & ~7ull : tiles;
+        const uint32_t gx = (uint32_t)std::min<uint64_t>(padded, 1u << 20);
+        const uint32_t gy = (uint32_t)((padded + gx - 1) / gx);
+        if (K == 6)
+            hipLaunchKernelGGL((k_bitrev_levels<T, 6, FOLD>), dim3(gx, gy), dim3(256), 0, s, A,
+                               t0, d, xcd, S);
+        else
+            hipLaunchKernelGGL((k_bitrev_levels<T, 5, FOLD>), dim3(gx, gy), dim3(256), 0, s, A,
+                               t0, d, xcd, S);
+    }
+    TL_LAUNCH_CHECK();
+    return TL_OK;
+}
+
 template <typename T>
 static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const uint64_t *blob,
                              T *stash, uint32_t *flags, bool preclear) {
@@ -427,7 +521,7 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             hipLaunchKernelGGL(k_lds_runs<T>, dim3((uint32_t)groups), dim3(threads), bytes, s, A,
                                runs, (uint32_t)op.nruns, op.nitems, (uint32_t)per, (uint32_t)n,
                                (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3], magic,
-                               (uint32_t)op.p[5]);
+                               (uint32_t)op.p[5], (uint32_t)op.p[6]);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -447,24 +541,34 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             const int64_t base = (int64_t)op.p[0];
             const uint32_t D = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
             const uint64_t tiles = 1ull << (D - 2 * K);
-            const uint32_t gx = (uint32_t)std::min<uint64_t>(tiles, 1u << 20);
-            const uint32_t gy = (uint32_t)((tiles + gx - 1) / gx);
-            if (K == 6)
-                hipLaunchKernelGGL((k_bitrev_tiles<T, 6>), dim3(gx, gy), dim3(256), 0, s, A, base, D);
+            const uint32_t per = (uint32_t)bitrev_grid_per_cu(), t0 = (uint32_t)op.p[4];
+            if (op.p[3] == BITREV_ABSORB)
+                TL_TRY((bitrev_launch<T, 4>(c, A, base, D, K, tiles, per, stash, t0, s)));
             else
-                hipLaunchKernelGGL((k_bitrev_tiles<T, 5>), dim3(gx, gy), dim3(256), 0, s, A, base, D);
-            TL_LAUNCH_CHECK();
+                TL_TRY((bitrev_launch<T, 0>(c, A, base, D, K, tiles, per, nullptr, 0, s)));
             break;
         }
         case OP_BITREV_LEVELS: {
             const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
-            const uint64_t tiles = (1ull << (d - t0)) - 1;
-            const uint32_t gx = (uint32_t)std::min<uint64_t>(tiles, 1u << 20);
-            const uint32_t gy = (uint32_t)((tiles + gx - 1) / gx);
-            if (K == 6)
-                hipLaunchKernelGGL((k_bitrev_levels<T, 6>), dim3(gx, gy), dim3(256), 0, s, A, t0, d);
+            const uint32_t per = (uint32_t)bitrev_grid_per_cu();
+            if (op.p[3] == BITREV_SHIFT)
+                TL_TRY((levels_launch<T, 1>(c, A, t0, d, K, per, stash, s)));
+            else if (op.p[3] == BITREV_UNFOLD)
+                TL_TRY((levels_launch<T, 2>(c, A, t0, d, K, per, stash, s)));
             else
-                hipLaunchKernelGGL((k_bitrev_levels<T, 5>), dim3(gx, gy), dim3(256), 0, s, A, t0, d);
+                TL_TRY((levels_launch<T, 0>(c, A, t0, d, K, per, nullptr, s)));
+            break;
+        }
+        case OP_HEADS: {
+            const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
+            const uint64_t total = rowend_slots(d, K, t0);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, c->num_cus * 64ull);
+            if (K == 6)
+                hipLaunchKernelGGL((k_level_heads<T, 6>), dim3(g), dim3(256), 0, s, (const T *)A,
+                                   stash, t0, d);
+            else
+                hipLaunchKernelGGL((k_level_heads<T, 5>), dim3(g), dim3(256), 0, s, (const T *)A,
+                                   stash, t0, d);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -474,8 +578,12 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             hipLaunchKernelGGL(k_copy_strided<T>, dim3(g), dim3(256), 0, s, stash, 1,
                                (const T *)(A + P), 1, w);
             TL_LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_rotate_walk<T, 4>), dim3((uint32_t)((w + 255) / 256)), dim3(256),
-                               0, s, A + P, L, k, w, (const T *)stash);
+            if (walk_depth() == 8)
+                hipLaunchKernelGGL((k_rotate_walk<T, 8>), dim3((uint32_t)((w + 255) / 256)),
+                                   dim3(256), 0, s, A + P, L, k, w, (const T *)stash);
+            else
+                hipLaunchKernelGGL((k_rotate_walk<T, 4>), dim3((uint32_t)((w + 255) / 256)),
+                                   dim3(256), 0, s, A + P, L, k, w, (const T *)stash);
             TL_LAUNCH_CHECK();
             break;
         }
@@ -521,7 +629,8 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
             const uint64_t words = ((nmoves + 15) / 16) * 16;
             if (!preclear) TL_HIP(hipMemsetAsync(op_flags, 0, (words + 16) * 4, s));
             unsigned long long *counter = (unsigned long long *)(op_flags + words);
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * 8);
+            const uint32_t grid =
+                (uint32_t)std::min<uint64_t>(nmoves, (uint64_t)c->num_cus * chained_wg_per_cu());
             if (op.ms.kind == MV_ROW_SPREAD)
                 hipLaunchKernelGGL((k_chained<T, CHAINED_PER, CH_ROW_SPREAD>), dim3(grid), dim3(256),
                                    0, s, A, op.ms, nmoves, op_flags, counter, c->err_flag,

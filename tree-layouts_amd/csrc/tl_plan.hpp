@@ -49,8 +49,39 @@ enum OpType {
     OP_COPY,
     OP_CHAINED,
     OP_BITREV_LEVELS,  // the per-level reversals of levels p[0] .. p[1]-1, tiles 2^p[2]
-    OP_ROTATE          // [p0, p0+p1) rotated right by p2 in one pass, p3 lanes (stash)
+    OP_ROTATE,         // [p0, p0+p1) rotated right by p2 in one pass, p3 lanes (stash)
+    OP_HEADS           // the heads of levels p[0] .. p[1]-1 (tiles 2^p[2]) to the stash
 };
+
+// A BST of 2^D keys has one leaf, the smallest key, which goes last; every
+// other key's final place is one left of where the tree kept one place right
+// (slot y at A[y]) has it, and that is where the sorted keys already are.
+// The plan keeps the tree there until its last phase, which writes every
+// level one slot left.  Forms of the bit-reversal launches (p[3]; tiles of
+// 2^K keys a side, t0 = 2K = p[4] for OP_BITREV):
+//   OP_BITREV, base 0: BITREV_HEAD (phase one of the kept tree: plain; its
+//     inverse is BITREV_ABSORB) or BITREV_ABSORB (the row ends at slots
+//     >= 2^t0 read from the stash, where BITREV_UNFOLD put them);
+//   OP_BITREV_LEVELS: BITREV_SHIFT (every level reversed in place and
+//     written one slot left; the first slots of the rows after each tile's
+//     own come from the heads that OP_HEADS put in the stash) or
+//     BITREV_UNFOLD (the inverse: level t read from A[2^t - 1 ..] and
+//     written to A[2^t ..], the row ends to the stash).
+// OP_HEADS has no inverse (the inverse plan drops it).  OP_LDS of kind
+// LDS_LEVELREV with p[6] = 1 reads its keys one place right (inverse: writes
+// them one place right).
+enum : uint64_t {
+    BITREV_PLAIN = 0,
+    BITREV_SHIFT = 1,
+    BITREV_UNFOLD = 2,
+    BITREV_HEAD = 3,
+    BITREV_ABSORB = 4
+};
+
+// stash slots of the row ends of levels t0 .. D-1 (rows of 2^K keys)
+inline uint64_t rowend_slots(uint32_t D, uint32_t K, uint32_t t0) {
+    return ((1ull << D) - (1ull << t0)) >> K;
+}
 
 struct Op {
     OpType type;
@@ -165,6 +196,7 @@ struct Tuning {
     bool fuse_leaves = true;               // vEB leaf layouts fused into streamed gathers
     uint64_t walk_lanes = 1ull << 24;      // lanes (stashed keys) of a one-pass rotation
     int64_t walk_env = -1;                 // $TL_WALK_LANES when set (0: no one-pass rotations)
+    bool bst_fold = true;                  // BST of 2^D keys: leaf move folded into phase one
 };
 
 inline void tuning_env(Tuning &t) {
@@ -174,6 +206,8 @@ inline void tuning_env(Tuning &t) {
     if (k && (atoi(k) == 5 || atoi(k) == 6)) t.bitrev_k = (uint32_t)atoi(k);
     const char *w = getenv("TL_WALK_LANES");
     if (w) t.walk_env = atoll(w);
+    const char *f = getenv("TL_BST_FOLD");
+    if (f && atoi(f) == 0) t.bst_fold = false;
 }
 
 inline uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -525,7 +559,42 @@ inline void plan_bst(Planner &P, uint64_t n) {
         P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
         return;
     }
-    uint32_t h = floor_log2_64(n);
+    const uint32_t h = floor_log2_64(n);
+    const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : 5u;
+    // n = 2^h: the one leaf (the smallest key) goes last and every other key
+    // one place left of the tree stored one place right.  As a gather that is
+    // a whole pass; folded into the last phase it is not.
+    if (P.tu.bst_fold && n == (1ull << h) && (n - 1) > P.cap() && h >= 2 * K + 2) {
+        const uint32_t t0 = 2 * K;
+        const uint64_t leaf = rowend_slots(h, K, t0);   // the leaf's stash slot
+        Op one;
+        one.type = OP_BITREV;
+        one.p[0] = 0;
+        one.p[1] = h;
+        one.p[2] = K;
+        one.p[3] = BITREV_HEAD;
+        one.p[4] = t0;
+        P.pl.max_stash = std::max<uint64_t>(P.pl.max_stash, leaf + 1);
+        P.pl.ops.push_back(one);
+        Op heads;
+        heads.type = OP_HEADS;
+        heads.p[0] = t0;
+        heads.p[1] = h;
+        heads.p[2] = K;
+        P.pl.ops.push_back(heads);
+        P.copy(0, 1, 1, true, leaf);
+        P.lds({{0, 0, 1}}, (1ull << t0) - 1, LDS_LEVELREV);
+        P.pl.ops.back().p[6] = 1;   // read one place right
+        Op two;
+        two.type = OP_BITREV_LEVELS;
+        two.p[0] = t0;
+        two.p[1] = h;
+        two.p[2] = K;
+        two.p[3] = BITREV_SHIFT;
+        P.pl.ops.push_back(two);
+        P.copy(n - 1, 1, 1, false, leaf);
+        return;
+    }
     uint32_t d;
     if (n != (1ull << (h + 1)) - 1) {
         const uint64_t nint = (1ull << h) - 1, nleaf = n - nint;
@@ -536,7 +605,6 @@ inline void plan_bst(Planner &P, uint64_t n) {
     } else {
         d = h + 1;
     }
-    const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : 5u;
     if (n <= P.cap() || d < 2 * K + 2) {
         P.lds({{0, 0, 1}}, n, LDS_BTREE, 1);
         return;
@@ -671,19 +739,39 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
         uint64_t end = 0;
         bool neg = false;
         switch (op.type) {
-        case OP_LDS: end = runs_end(pl, op, op.p[0]); break;
+        case OP_LDS:
+            end = runs_end(pl, op, op.p[0]) + (op.p[6] ? 1 : 0);
+            if (op.p[6] && (op.p[1] != LDS_LEVELREV || op.p[6] != 1)) neg = true;
+            break;
         case OP_REVERSE: end = runs_end(pl, op, op.p[0]); break;
         case OP_BITREV: {
             const int64_t base = (int64_t)op.p[0];
             const uint32_t D = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
             if (base < -1 || D < 2 * K || D > 62) neg = true;
             end = (uint64_t)(base + (int64_t)(1ull << D));
+            if (op.p[3] == BITREV_HEAD || op.p[3] == BITREV_ABSORB) {
+                if (base != 0 || op.p[4] != 2 * K || rowend_slots(D, K, 2 * K) > pl.max_stash)
+                    neg = true;
+            } else if (op.p[3] != BITREV_PLAIN) {
+                neg = true;
+            }
             break;
         }
         case OP_BITREV_LEVELS: {
             const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
-            if (t0 != 2 * K || d <= t0 || d > 62) neg = true;
+            if (t0 != 2 * K || d <= t0 || d > 62 || op.p[3] > BITREV_UNFOLD) neg = true;
             end = (1ull << d) - 1;
+            if (op.p[3] != BITREV_PLAIN) {   // one place right: up to A[2^d - 1]
+                end = 1ull << d;
+                if (rowend_slots(d, K, t0) > pl.max_stash) neg = true;
+            }
+            break;
+        }
+        case OP_HEADS: {
+            const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
+            if (t0 != 2 * K || d <= t0 || d > 62 || rowend_slots(d, K, t0) > pl.max_stash)
+                neg = true;
+            end = 1ull << d;
             break;
         }
         case OP_ROTATE:
@@ -765,8 +853,11 @@ constexpr uint32_t PLAN_INVERSE = 0x100;
 
 // The inverse of a forward plan (layout -> sorted order): the launches in
 // reverse order, each replaced by its inverse.  Reversals and bit-reversal
-// tiles are involutions; LDS layouts, unit merges and streamed root gathers
-// run with their inverse flag; copies swap direction; a shift becomes the
+// tiles are involutions (for a BST of 2^D keys the shifted level reversal
+// becomes the unfold, which puts the row ends in the stash, and phase one
+// then absorbs them from there; the heads copy is dropped); LDS layouts, unit
+// merges and streamed root gathers run with their inverse flag; copies swap
+// direction; a shift becomes the
 // opposite shift of the moved region; a one-pass rotation by k becomes the
 // rotation by len - k (it fills its own stash); a row slide becomes a row spread (which
 // takes the separators back from the stash the inverted copy before it
@@ -785,9 +876,14 @@ inline void invert_plan(const Plan &f, Plan &g) {
         case OP_UNIT_MERGE: op.p[4] ^= 1; break;
         case OP_SPARSE_STREAM: op.p[2] ^= 1; break;
         case OP_COPY: op.p[3] ^= 1; break;
-        case OP_REVERSE:
-        case OP_BITREV:
-        case OP_BITREV_LEVELS: break;
+        case OP_REVERSE: break;
+        case OP_BITREV:   // an involution; the kept tree's phase one absorbs row ends
+            if (op.p[3] != BITREV_PLAIN) op.p[3] = BITREV_HEAD + BITREV_ABSORB - op.p[3];
+            break;
+        case OP_BITREV_LEVELS:   // the shift becomes the unfold
+            if (op.p[3] != BITREV_PLAIN) op.p[3] = BITREV_SHIFT + BITREV_UNFOLD - op.p[3];
+            break;
+        case OP_HEADS: continue;   // the unfold keeps row ends instead
         case OP_ROTATE: op.p[2] = op.p[1] - op.p[2]; break;
         case OP_CHAINED: {
             MoveSpec &m = op.ms;

@@ -10,6 +10,7 @@
 
 #include "tl_internal.hpp"
 #include "tl_descent.hpp"
+#include "tl_queue.hpp"
 
 namespace tl {
 
@@ -22,25 +23,22 @@ template <typename K>
 constexpr uint32_t BST_TOP = sizeof(K) == 4 ? 4095 : 2047;
 constexpr int BST_QPL = 4;   // queries in flight per lane
 
+// queries first, first + stride, ... below end of one array: the body of the
+// one-array kernel (a grid stride) and of the batch's query queue (a chunk
+// of one workgroup)
 template <typename K>
-__global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint64_t n,
-                                                   const K *__restrict__ Q, uint64_t nq,
-                                                   uint64_t *__restrict__ ans) {
-    __shared__ K top[BST_TOP<K>];
-    const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP<K>, n);
-    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
-    __syncthreads();
-    // BST_QPL independent queries per lane, strided by the grid for coalescing
-    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
-         q0 += all * BST_QPL) {
+__device__ inline void bst_body(const K *__restrict__ A, uint64_t n, const K *top, uint32_t ntop,
+                                const K *__restrict__ Q, uint64_t *__restrict__ ans,
+                                uint64_t first, uint64_t end, uint64_t stride) {
+    // BST_QPL independent queries per lane, `stride` apart for coalescing
+    for (uint64_t q0 = first; q0 < end; q0 += stride * BST_QPL) {
         K qv[BST_QPL];
         uint64_t iv[BST_QPL], rv[BST_QPL];
 #pragma unroll
         for (int k = 0; k < BST_QPL; k++) {
-            const uint64_t qi = q0 + k * all;
-            qv[k] = qi < nq ? Q[qi] : K{};
-            iv[k] = qi < nq ? 0 : n;
+            const uint64_t qi = q0 + k * stride;
+            qv[k] = qi < end ? Q[qi] : K{};
+            iv[k] = qi < end ? 0 : n;
             rv[k] = n;
         }
         bool any = true;
@@ -65,10 +63,28 @@ __global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint
         }
 #pragma unroll
         for (int k = 0; k < BST_QPL; k++) {
-            const uint64_t qi = q0 + k * all;
-            if (qi < nq) ans[qi] = rv[k];
+            const uint64_t qi = q0 + k * stride;
+            if (qi < end) ans[qi] = rv[k];
         }
     }
+}
+
+template <typename K>
+__device__ inline uint32_t bst_stage(const K *__restrict__ A, uint64_t n, K *top) {
+    const uint32_t ntop = (uint32_t)min((uint64_t)BST_TOP<K>, n);
+    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
+    __syncthreads();
+    return ntop;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_query_bst(const K *__restrict__ A, uint64_t n,
+                                                   const K *__restrict__ Q, uint64_t nq,
+                                                   uint64_t *__restrict__ ans) {
+    __shared__ K top[BST_TOP<K>];
+    const uint32_t ntop = bst_stage<K>(A, n, top);
+    bst_body<K>(A, n, top, ntop, Q, ans, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nq,
+                (uint64_t)gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -103,18 +119,19 @@ __global__ __launch_bounds__(256) void k_query_btree_group(const K *__restrict__
 }
 
 // B-tree, wide form (descent_wide): groups of G lanes holding 16 bytes of
-// the node each, QN queries per group.
+// the node each, QN queries per group.  The body answers queries lo + first,
+// + groups, ... below hi; every lane of a workgroup makes the same number of
+// trips (the descent's ballots need the whole wave).
 template <typename K, int G, int QN, typename IDX>
-__global__ __launch_bounds__(256) void k_query_btree_wide(const K *__restrict__ A, uint64_t n,
-                                                          uint32_t b, const K *__restrict__ Q,
-                                                          uint64_t nq, uint64_t *__restrict__ ans) {
+__device__ inline void btree_wide_body(const K *__restrict__ A, uint64_t n, uint32_t b,
+                                       const K *__restrict__ Q, uint64_t *__restrict__ ans,
+                                       uint64_t lo, uint64_t hi, uint64_t first, uint64_t groups) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t j = lane % G;
     const uint64_t gm = group_mask<G>(lane);
     const IDX nodes = (IDX)((n + b - 1) / b);
-    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
-    const uint64_t trips = (nq + QN * groups - 1) / (QN * groups);   // uniform trip count
-    uint64_t q0 = (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    const uint64_t trips = (hi - lo + QN * groups - 1) / (QN * groups);   // uniform trip count
+    uint64_t q0 = lo + first;
     for (uint64_t it = 0; it < trips; it++, q0 += QN * groups) {
         K q[QN];
         IDX node[QN];
@@ -123,7 +140,7 @@ __global__ __launch_bounds__(256) void k_query_btree_wide(const K *__restrict__ 
 #pragma unroll
         for (int k = 0; k < QN; k++) {
             const uint64_t qi = q0 + k * groups;
-            act[k] = qi < nq;
+            act[k] = qi < hi;
             q[k] = act[k] ? Q[qi] : K{};
             node[k] = 0;
         }
@@ -132,10 +149,20 @@ __global__ __launch_bounds__(256) void k_query_btree_wide(const K *__restrict__ 
 #pragma unroll
             for (int k = 0; k < QN; k++) {
                 const uint64_t qi = q0 + k * groups;
-                if (qi < nq) ans[qi] = res[k];
+                if (qi < hi) ans[qi] = res[k];
             }
         }
     }
+}
+
+template <typename K, int G, int QN, typename IDX>
+__global__ __launch_bounds__(256) void k_query_btree_wide(const K *__restrict__ A, uint64_t n,
+                                                          uint32_t b, const K *__restrict__ Q,
+                                                          uint64_t nq, uint64_t *__restrict__ ans) {
+    const uint64_t groups = (uint64_t)gridDim.x * (blockDim.x / G);
+    btree_wide_body<K, G, QN, IDX>(A, n, b, Q, ans, 0, nq,
+                                   (uint64_t)blockIdx.x * (blockDim.x / G) + threadIdx.x / G,
+                                   groups);
 }
 
 // B-tree, lane form (b == 1 or b > 64): the reference's linear node scan
@@ -304,38 +331,47 @@ __host__ __device__ inline VebSmem veb_smem(uint32_t table_size, uint32_t ntop, 
     return m;
 }
 
+// the LDS copies a workgroup works from: tables with every entry's pos[] slot,
+// idx, table_idx, the top keys, and the pos[] slots of its lanes
+template <typename K, typename PT>
+struct VebView {
+    TblEnt *tb;
+    uint64_t *s_idx;
+    uint32_t *s_tidx;
+    K *top;
+    PT *pos;
+};
+
 template <typename K, typename PT, int QPL>
-__global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uint64_t n,
-                                                    const tl_veb_table *__restrict__ tables,
-                                                    uint32_t table_size,
-                                                    const uint64_t *__restrict__ idx,
-                                                    const uint32_t *__restrict__ tidx,
-                                                    uint32_t ntables, uint32_t ntop,
-                                                    const K *__restrict__ Q, uint64_t nq,
-                                                    uint64_t *__restrict__ ans) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+__device__ inline VebView<K, PT> veb_stage(unsigned char *smem_raw, const K *__restrict__ A,
+                                           uint64_t n, const tl_veb_table *__restrict__ tables,
+                                           uint32_t table_size, const uint64_t *__restrict__ idx,
+                                           const uint32_t *__restrict__ tidx, uint32_t ntables,
+                                           uint32_t ntop) {
     const VebSmem lay = veb_smem(table_size, ntop, sizeof(K), sizeof(PT), QPL);
-    TblEnt *tb = reinterpret_cast<TblEnt *>(smem_raw);
-    uint64_t *s_idx = reinterpret_cast<uint64_t *>(tb + table_size);
-    uint32_t *s_tidx = reinterpret_cast<uint32_t *>(s_idx + 64);
-    K *top = reinterpret_cast<K *>(smem_raw + lay.top);
-    PT *pos = reinterpret_cast<PT *>(smem_raw + lay.pos);
+    VebView<K, PT> v;
+    v.tb = reinterpret_cast<TblEnt *>(smem_raw);
+    v.s_idx = reinterpret_cast<uint64_t *>(v.tb + table_size);
+    v.s_tidx = reinterpret_cast<uint32_t *>(v.s_idx + 64);
+    v.top = reinterpret_cast<K *>(smem_raw + lay.top);
+    v.pos = reinterpret_cast<PT *>(smem_raw + lay.pos);
+    TblEnt *tb = v.tb;
     for (uint32_t i = threadIdx.x; i < table_size; i += blockDim.x) {
         tb[i].L = tables[i].L;
         tb[i].R = tables[i].R;
         tb[i].D = tables[i].D;
     }
     for (uint32_t i = threadIdx.x; i < ntables; i += blockDim.x) {
-        s_idx[i] = idx ? idx[i] : n;
-        s_tidx[i] = tidx ? tidx[i] : 0;
+        v.s_idx[i] = idx ? idx[i] : n;
+        v.s_tidx[i] = tidx ? tidx[i] : 0;
     }
-    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = A[i];
+    for (uint32_t i = threadIdx.x; i < ntop; i += blockDim.x) v.top[i] = A[i];
     __syncthreads();
     // slot of every entry: the length of its chain t -> D[t] -> .. -> 0
     for (uint32_t i = threadIdx.x; i < table_size; i += blockDim.x) {
         uint32_t base = 0;
         for (uint32_t j = 0; j < ntables; j++)
-            if (s_tidx[j] <= i) base = max(base, s_tidx[j]);
+            if (v.s_tidx[j] <= i) base = max(base, v.s_tidx[j]);
         uint32_t x = i - base, sl = 0;
         for (uint32_t it = 0; x != 0 && it < 64; it++) {
             const uint32_t nx = tb[base + x].D;
@@ -346,13 +382,25 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
         tb[i].slot = min(max(sl, 1u), VEB_SLOTS - 1);   // entry 0 (the root) is never read
     }
     __syncthreads();
+    return v;
+}
+
+// queries first, first + stride, ... below end
+template <typename K, typename PT, int QPL>
+__device__ inline void veb_body(const VebView<K, PT> &v, const K *__restrict__ A, uint64_t n,
+                                uint32_t ntables, uint32_t ntop, const K *__restrict__ Q,
+                                uint64_t *__restrict__ ans, uint64_t first, uint64_t end,
+                                uint64_t stride) {
+    const TblEnt *tb = v.tb;
+    const uint64_t *s_idx = v.s_idx;
+    const uint32_t *s_tidx = v.s_tidx;
+    const K *top = v.top;
+    PT *pos = v.pos;
     const uint32_t W = blockDim.x, me = threadIdx.x;
     auto P = [&](uint32_t slot, int k) -> PT & { return pos[(slot * QPL + k) * W + me]; };
-    const uint64_t all = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t q0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nq;
-         q0 += all * QPL) {
+    for (uint64_t q0 = first; q0 < end; q0 += stride * QPL) {
         K q[QPL];
-        uint64_t res[QPL], offset[QPL], end[QPL], bfs[QPL], index[QPL];
+        uint64_t res[QPL], offset[QPL], endj[QPL], bfs[QPL], index[QPL];
         uint32_t j[QPL], cd[QPL], dj[QPL], tbase[QPL];
         bool act[QPL];
         // structure j of query k: from its root, or the chain's end
@@ -362,8 +410,8 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
                     act[k] = false;
                     break;
                 }
-                end[k] = s_idx[j[k]];
-                if (offset[k] < end[k]) {
+                endj[k] = s_idx[j[k]];
+                if (offset[k] < endj[k]) {
                     tbase[k] = s_tidx[j[k]];
                     dj[k] = floor_log2_64(n - offset[k]) + 1;
                     bfs[k] = 1;
@@ -372,14 +420,14 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
                     P(0, k) = 1;
                     break;
                 }
-                offset[k] = end[k];
+                offset[k] = endj[k];
                 j[k]++;
             }
         };
 #pragma unroll
         for (int k = 0; k < QPL; k++) {
-            const uint64_t qi = q0 + k * all;
-            act[k] = qi < nq;
+            const uint64_t qi = q0 + k * stride;
+            act[k] = qi < end;
             q[k] = act[k] ? Q[qi] : K{};
             res[k] = n;
             offset[k] = 0;
@@ -415,8 +463,8 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
                                 (uint64_t)P(e.slot - 1, k) + e.R + (bfs[k] & e.R) * e.L;
                             P(e.slot, k) = (PT)p;
                             index[k] = p - 1;
-                            if (index[k] + offset[k] >= end[k]) {
-                                offset[k] = end[k];
+                            if (index[k] + offset[k] >= endj[k]) {
+                                offset[k] = endj[k];
                                 j[k]++;
                                 enter(k);
                             }
@@ -428,10 +476,177 @@ __global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uin
         }
 #pragma unroll
         for (int k = 0; k < QPL; k++) {
-            const uint64_t qi = q0 + k * all;
-            if (qi < nq) ans[qi] = res[k];
+            const uint64_t qi = q0 + k * stride;
+            if (qi < end) ans[qi] = res[k];
         }
     }
+}
+
+template <typename K, typename PT, int QPL>
+__global__ __launch_bounds__(256) void k_query_mveb(const K *__restrict__ A, uint64_t n,
+                                                    const tl_veb_table *__restrict__ tables,
+                                                    uint32_t table_size,
+                                                    const uint64_t *__restrict__ idx,
+                                                    const uint32_t *__restrict__ tidx,
+                                                    uint32_t ntables, uint32_t ntop,
+                                                    const K *__restrict__ Q, uint64_t nq,
+                                                    uint64_t *__restrict__ ans) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const VebView<K, PT> v =
+        veb_stage<K, PT, QPL>(smem_raw, A, n, tables, table_size, idx, tidx, ntables, ntop);
+    veb_body<K, PT, QPL>(v, A, n, ntables, ntop, Q, ans,
+                         (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nq,
+                         (uint64_t)gridDim.x * blockDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// the uneven batch's query queue: one persistent launch answers every array's
+// queries.  The work is cut into chunks of `chunk` queries of one array;
+// workgroups take chunk numbers from an atomic ticket until none are left (no
+// workgroup ever waits for another), find the chunk's array by a binary search
+// of the chunks' prefix sums, stage what that array's descent keeps in LDS
+// and answer the chunk with the body of the one-array kernel.
+constexpr int QQ_QN = 4;    // B-tree descents per lane group
+constexpr int QQ_QPL = 2;   // mvEB queries per lane
+
+__device__ inline uint32_t qq_find(const uint64_t *__restrict__ pre, uint32_t nd, uint64_t t) {
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename K, int LAYOUT>
+__device__ inline void qq_chunk(const QueueItem &it, uint64_t lo, uint64_t hi,
+                                unsigned char *smem) {
+    const K *A = (const K *)it.A;
+    const K *Q = (const K *)it.Q;
+    uint64_t *ans = (uint64_t *)it.ans;
+    if constexpr (LAYOUT == 0) {
+        K *top = reinterpret_cast<K *>(smem);
+        const uint32_t ntop = bst_stage<K>(A, it.n, top);
+        bst_body<K>(A, it.n, top, ntop, Q, ans, lo + threadIdx.x, hi, blockDim.x);
+    } else if constexpr (LAYOUT == 1) {
+        switch (it.G) {
+#define TL_QW(GG)                                                                                  \
+    case GG:                                                                                       \
+        btree_wide_body<K, GG, QQ_QN, uint32_t>(A, it.n, it.b, Q, ans, lo, hi, threadIdx.x / GG,   \
+                                                blockDim.x / GG);                                  \
+        break;
+            TL_QW(1) TL_QW(2) TL_QW(4) TL_QW(8) TL_QW(16)
+#undef TL_QW
+        default: break;
+        }
+    } else {
+        const VebView<K, uint32_t> v = veb_stage<K, uint32_t, QQ_QPL>(
+            smem, A, it.n, (const tl_veb_table *)it.tables, it.table_size,
+            (const uint64_t *)it.idx, (const uint32_t *)it.tidx, it.ntables, it.ntop);
+        veb_body<K, uint32_t, QQ_QPL>(v, A, it.n, it.ntables, it.ntop, Q, ans, lo + threadIdx.x,
+                                      hi, blockDim.x);
+    }
+}
+
+// one queue per layout, so that each kernel holds only its own descent's
+// registers
+template <int LAYOUT>
+__global__ __launch_bounds__(256) void k_query_queue(const QueueItem *__restrict__ items,
+                                                     const uint64_t *__restrict__ pre,
+                                                     uint32_t nitems, uint64_t chunks,
+                                                     uint64_t chunk,
+                                                     unsigned long long *ticket) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ uint64_t s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const uint64_t t = s_t;
+        __syncthreads();
+        if (t >= chunks) return;
+        const uint32_t i = qq_find(pre, nitems, t);
+        const QueueItem it = items[i];
+        const uint64_t lo = (t - pre[i]) * chunk;
+        const uint64_t hi = min(it.nq, lo + chunk);
+        switch (it.dtype) {
+        case TL_U32: qq_chunk<uint32_t, LAYOUT>(it, lo, hi, smem_raw); break;
+        case TL_U64: qq_chunk<uint64_t, LAYOUT>(it, lo, hi, smem_raw); break;
+        case TL_I32: qq_chunk<int32_t, LAYOUT>(it, lo, hi, smem_raw); break;
+        default: qq_chunk<int64_t, LAYOUT>(it, lo, hi, smem_raw); break;
+        }
+        __syncthreads();   // the next chunk restages the LDS
+    }
+}
+
+bool queue_takes(const tl_batch_item &it) {
+    if (it.nq == 0 || it.n == 0 || it.n >= (1ull << 32)) return false;
+    if (it.layout != 1) return it.layout <= 2;
+    const uint32_t w = (it.dtype == TL_U32 || it.dtype == TL_I32) ? 4 : 8;
+    return wide_ok(it.A, it.b, w) && wide_narrow(it.n, it.b) && wide_group(it.b, w) <= 16;
+}
+
+QueueItem queue_item(const tl_batch_item &it, uint64_t tables, uint64_t idx, uint64_t tidx,
+                     uint32_t ntables, uint32_t table_size, size_t *lds) {
+    const uint32_t w = (it.dtype == TL_U32 || it.dtype == TL_I32) ? 4 : 8;
+    QueueItem q{};
+    q.A = (uint64_t)it.A;
+    q.Q = (uint64_t)it.Q;
+    q.ans = (uint64_t)it.ans;
+    q.n = it.n;
+    q.nq = it.nq;
+    q.dtype = (uint32_t)it.dtype;
+    q.layout = it.layout;
+    q.b = (uint32_t)it.b;
+    size_t need = 0;
+    if (it.layout == 0) {
+        need = (size_t)(w == 4 ? BST_TOP<uint32_t> : BST_TOP<uint64_t>) * w;
+    } else if (it.layout == 1) {
+        q.G = wide_group(it.b, w);
+    } else {
+        q.tables = tables;
+        q.idx = idx;
+        q.tidx = tidx;
+        q.ntables = ntables;
+        q.table_size = table_size;
+        q.ntop = (uint32_t)std::min<uint64_t>(it.n, w == 4 ? 4095u : VEB_TOP);
+        need = veb_smem(table_size, q.ntop, w, 4, QQ_QPL).bytes;
+        if (need > 64 * 1024) {   // a long table chain: no staged top keys
+            q.ntop = 0;
+            need = veb_smem(table_size, 0, w, 4, QQ_QPL).bytes;
+        }
+    }
+    *lds = need;
+    return q;
+}
+
+uint64_t queue_chunk() {
+    const char *e = getenv("TL_QQ_CHUNK");
+    const long long v = e ? atoll(e) : 0;
+    return (v >= 1024 && v <= (1 << 20)) ? (uint64_t)v : 4096u;
+}
+
+tl_status queue_launch(Ctx *c, uint32_t layout, const QueueItem *items, const uint64_t *pre,
+                       uint32_t nitems, uint64_t chunks, uint64_t chunk,
+                       unsigned long long *ticket, size_t lds, hipStream_t s) {
+    if (!nitems || !chunks) return TL_OK;
+    const char *e = getenv("TL_QQ_GRID");
+    const int per = e ? atoi(e) : 16;
+    const uint64_t wg = (per >= 1 && per <= 64) ? (uint64_t)per : 16u;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(chunks, (uint64_t)c->num_cus * wg);
+    if (layout == 0)
+        hipLaunchKernelGGL(k_query_queue<0>, dim3(grid), dim3(256), lds, s, items, pre, nitems,
+                           chunks, chunk, ticket);
+    else if (layout == 1)
+        hipLaunchKernelGGL(k_query_queue<1>, dim3(grid), dim3(256), lds, s, items, pre, nitems,
+                           chunks, chunk, ticket);
+    else
+        hipLaunchKernelGGL(k_query_queue<2>, dim3(grid), dim3(256), lds, s, items, pre, nitems,
+                           chunks, chunk, ticket);
+    TL_LAUNCH_CHECK();
+    return TL_OK;
 }
 
 // mvEB query shape: $TL_VEB_QPL (1, 2, 4) queries per lane and $TL_VEB_TOP

@@ -124,9 +124,9 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [("lds_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
                 ("wait_ms", ctypes.c_uint32), ("tile_k", ctypes.c_uint32),
                 ("stream_threads", ctypes.c_uint32), ("btree_qpg", ctypes.c_uint32),
-                ("veb_qpl", ctypes.c_uint32)]
+                ("veb_qpl", ctypes.c_uint32), ("query_queue", ctypes.c_uint32)]
 
-BATCH_KNOBS = ("lds_bytes", "tile_k", "stream_threads", "btree_qpg", "veb_qpl")
+BATCH_KNOBS = ("lds_bytes", "tile_k", "stream_threads", "btree_qpg", "veb_qpl", "query_queue")
 
 
 _OPTIONAL = {
@@ -524,7 +524,7 @@ class Batch:
     b, Q (tensor or None), ans (int64 tensor or None)."""
 
     def __init__(self, arrays, lds_bytes=0, wg_per_cu=0, wait_ms=0, tile_k=0, stream_threads=0,
-                 btree_qpg=0, veb_qpl=0):
+                 btree_qpg=0, veb_qpl=0, query_queue=0):
         self.arrays = arrays
         items = (BatchItem * len(arrays))()
         for i, a in enumerate(arrays):
@@ -538,7 +538,8 @@ class Batch:
             items[i].nq = q.numel() if q is not None else 0
             items[i].ans = a["ans"].data_ptr() if q is not None else None
         self._items = items
-        cfg = BatchConfig(lds_bytes, wg_per_cu, wait_ms, tile_k, stream_threads, btree_qpg, veb_qpl)
+        cfg = BatchConfig(lds_bytes, wg_per_cu, wait_ms, tile_k, stream_threads, btree_qpg, veb_qpl,
+                          query_queue)
         h = ctypes.c_void_p()
         _check(lib().tl_batch_create(items, len(arrays), ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
