@@ -26,7 +26,9 @@ CHAINED_KEYS = 256 * 32   # keys of one chained move (32 per thread)
 class MoveSpec(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("P", ctypes.c_uint64), ("L", ctypes.c_uint64),
                 ("k", ctypes.c_uint64), ("r", ctypes.c_uint64), ("l", ctypes.c_uint64),
-                ("CH", ctypes.c_uint64), ("nchunk", ctypes.c_uint64)]
+                ("CH", ctypes.c_uint64), ("nchunk", ctypes.c_uint64),
+                ("fuse", ctypes.c_uint32), ("rp", ctypes.c_uint64), ("lp", ctypes.c_uint64),
+                ("soff", ctypes.c_uint64)]
 
 
 class BOp(ctypes.Structure):
@@ -99,10 +101,12 @@ def move_at(m, t):
         j0 = t * m.CH
         j1 = min(m.r, j0 + m.CH)
         return m.P + m.r + j0 * m.l, m.P + j0 * (m.l + 1), (j1 - j0) * m.l
+    fuse = getattr(m, "fuse", 0)
     if m.kind == pe.MV_SPARSE_SPREAD:
         if t < m.r * m.nchunk:
             i, c = divmod(t, m.nchunk)
-            return (m.P + m.r + m.k + i * m.l + c * m.CH, m.P + i * (m.l + 1) + c * m.CH,
+            off = m.rp + c * m.CH // (m.lp + 1) * m.lp if fuse else c * m.CH
+            return (m.P + m.r + m.k + i * m.l + off, m.P + i * (m.l + 1) + c * m.CH,
                     min(m.CH, m.l - c * m.CH))
         c = t - m.r * m.nchunk
         dst = m.P + m.r * (m.l + 1) + c * m.CH
@@ -116,7 +120,8 @@ def move_at(m, t):
     i = m.r - 1 - t // m.nchunk
     c = m.nchunk - 1 - t % m.nchunk
     src = m.P + i * (m.l + 1) + c * m.CH
-    return src, m.P + m.r + m.k + i * m.l + c * m.CH, min(m.CH, m.l - c * m.CH)
+    off = m.rp + c * m.CH // (m.lp + 1) * m.lp if fuse else c * m.CH
+    return src, m.P + m.r + m.k + i * m.l + off, min(m.CH, m.l - c * m.CH)
 
 
 def awaited_moves(m, t, dst, ln):
@@ -147,7 +152,11 @@ def awaited_moves(m, t, dst, ln):
             def num(x):
                 if x < leaf:
                     i = x // m.l
-                    return i * m.nchunk + (x - i * m.l) // m.CH
+                    w = x - i * m.l
+                    if not getattr(m, "fuse", 0):
+                        return i * m.nchunk + w // m.CH
+                    rows = 0 if w < m.rp else (w - m.rp) // m.lp
+                    return i * m.nchunk + rows // (m.CH // (m.lp + 1))
                 return m.r * m.nchunk + (x - leaf) // m.CH
             lo, hi = num(a), num(z)
         return [u for u in range(lo, hi + 1) if u < t]

@@ -96,9 +96,11 @@ static void lane_main(int l) {
 
 void emu_run_block(dim3 grid, dim3 block, dim3 bid, size_t smem, const std::function<void()> &body) {
     const int nl = (int)(block.x * block.y * block.z);
-    if (g_pool.empty())
-        for (int l = 0; l < 256; l++) g_pool.emplace_back(lane_main, l), g_pool.back().detach();
-    if (nl > 256) { fprintf(stderr, "EMU: block of %d lanes\n", nl); abort(); }
+    if (nl > 1024) { fprintf(stderr, "EMU: block of %d lanes\n", nl); abort(); }
+    // grow the pool to the widest workgroup seen (the new lanes wait for the
+    // next job like the others)
+    for (int l = (int)g_pool.size(); l < std::max(nl, 256); l++)
+        g_pool.emplace_back(lane_main, l), g_pool.back().detach();
     g_block.reset(nl);
     for (int w = 0; w < (nl + 63) / 64; w++) g_wave[w].reset(std::min(64, nl - 64 * w));
     g_smem_bytes = smem;

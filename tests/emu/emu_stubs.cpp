@@ -42,6 +42,7 @@ tl_status check_err_flag(Ctx *c, hipStream_t) {
 tl_status Timer::start(hipStream_t, float *m) { ms = m; return TL_OK; }
 tl_status Timer::stop() { if (ms) *ms = 0; return TL_OK; }
 Timer::~Timer() {}
+#ifndef EMU_WITH_PERMUTE   // the emulated tl_permute.hip defines the launches itself
 tl_status fill_u32_launch(uint32_t *p, uint64_t count, uint32_t v, hipStream_t) {
     for (uint64_t i = 0; i < count; i++) p[i] = v;
     return TL_OK;
@@ -55,6 +56,7 @@ tl_status lds_items_launch(const uint64_t *, uint32_t, uint32_t, uint32_t, uint3
 tl_status multi_launch(Ctx *, const MultiLaunch &, hipStream_t) {
     fprintf(stderr, "[emu] multi_launch not expected\n"); abort();
 }
+#endif
 }   // namespace tl
 #define NOTEXP(name) { fprintf(stderr, "[emu] " #name " not expected\n"); abort(); }
 extern "C" {
@@ -62,9 +64,11 @@ const char *tl_last_error(void) { return "see [emu] set_error lines"; }
 tl_status tl_query_bst(const void *, uint64_t, tl_dtype, const void *, uint64_t, uint64_t *, void *, float *) NOTEXP(tl_query_bst)
 tl_status tl_query_btree(const void *, uint64_t, uint64_t, tl_dtype, const void *, uint64_t, uint64_t *, void *, float *) NOTEXP(tl_query_btree)
 tl_status tl_query_mveb_nonperfect(const void *, const tl_veb_table *, const uint64_t *, const uint32_t *, uint64_t, uint32_t, uint32_t, tl_dtype, const void *, uint64_t, uint64_t *, void *, float *) NOTEXP(tl_query_mveb_nonperfect)
+#ifndef EMU_WITH_PERMUTE
 tl_status tl_permute_bst(void *, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_permute_bst)
 tl_status tl_permute_btree(void *, uint64_t, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_permute_btree)
 tl_status tl_permute_mveb(void *, uint64_t, tl_dtype, void *, float *) NOTEXP(tl_permute_mveb)
+#endif
 }
 
 #include "src/tl_queue.hpp"

@@ -10,7 +10,7 @@ import numpy as np
 import oracle_lib as ol
 
 (OP_LDS, OP_REVERSE, OP_BITREV, OP_UNIT_MERGE, OP_SPARSE_STREAM, OP_COPY, OP_CHAINED,
- OP_BITREV_LEVELS, OP_ROTATE, OP_HEADS) = range(10)
+ OP_BITREV_LEVELS, OP_ROTATE, OP_HEADS, OP_PART_ROOTS) = range(11)
 BITREV_PLAIN, BITREV_SHIFT, BITREV_UNFOLD, BITREV_HEAD, BITREV_ABSORB = range(5)
 LDS_BTREE, LDS_MVEB, LDS_GATHER, LDS_LEVELREV, LDS_VEB = range(5)
 MV_SHIFT_RIGHT, MV_SHIFT_LEFT, MV_SPARSE_SLIDE, MV_ROW_SLIDE, MV_ROW_SPREAD, MV_SPARSE_SPREAD = \
@@ -90,6 +90,33 @@ def _tree_row_ends(D, K, t0):
         ends.append((1 << lev) + e)
         slots.append(_level_slot(lev, e, K, t0))
     return np.concatenate(ends), np.concatenate(slots)
+
+
+def _fuse_blocks(blocks, fuse, rp, lp):
+    """leaf blocks (rows of rp rows of (lp keys, 1 root) and lp keys) ->
+    their rp + 1 lp-key blocks, each in vEB order"""
+    r = blocks.shape[0]
+    pos = lds_perm(LDS_MVEB if fuse == 1 else LDS_VEB, lp, 0, 0)
+    rows = blocks[:, :rp * (lp + 1)].reshape(r, rp, lp + 1)[:, :, :lp]
+    leaf = np.concatenate([rows, blocks[:, rp * (lp + 1):].reshape(r, 1, lp)], axis=1)
+    out = np.empty_like(leaf)
+    out[:, :, pos] = leaf
+    return out.reshape(r, (rp + 1) * lp)
+
+
+def _unfuse_blocks(final, fuse, rp, lp, stash, soff):
+    """the inverse of _fuse_blocks on the final blocks (rp roots' places,
+    then rp + 1 lp-key blocks in vEB order), with the row roots from the
+    stash"""
+    r = final.shape[0]
+    pos = lds_perm(LDS_MVEB if fuse == 1 else LDS_VEB, lp, 0, 0)
+    leaf = final[:, rp:].reshape(r, rp + 1, lp)[:, :, pos]
+    out = np.empty((r, rp * (lp + 1) + lp), dtype=final.dtype)
+    rows = out[:, :rp * (lp + 1)].reshape(r, rp, lp + 1)
+    rows[:, :, :lp] = leaf[:, :rp]
+    rows[:, :, lp] = stash[soff: soff + r * rp].reshape(r, rp)
+    out[:, rp * (lp + 1):] = leaf[:, rp]
+    return out
 
 
 def replay(buf, A, stash=None):
@@ -239,7 +266,10 @@ def replay(buf, A, stash=None):
                 reg[:, l] = stash[:r]
             elif kind == MV_SPARSE_SPREAD:
                 T0 = P + r * (l + 1)
+                fuse, rp, lp, soff = (int(v) for v in op[20:24])
                 leaves = A[P + r + k: P + r + k + r * l].reshape(r, l).copy()
+                if fuse:   # each leaf block's own gather undone, roots from the stash
+                    leaves = _unfuse_blocks(leaves, fuse, rp, lp, stash, soff)
                 tail = A[T0 + k: T0 + k + L].copy()
                 A[P:P + r * (l + 1)].reshape(r, l + 1)[:, :l] = leaves
                 A[T0:T0 + L] = tail
@@ -251,10 +281,31 @@ def replay(buf, A, stash=None):
             else:
                 # rows of (l leaf keys, 1 root) then a tail of L keys, all slid
                 # right past the r roots and k extra roots (both stashed before)
+                fuse, rp, lp, soff = (int(v) for v in op[20:24])
                 reg = A[P:P + r * (l + 1)].reshape(r, l + 1).copy()
                 tail = A[P + r * (l + 1): P + r * (l + 1) + L].copy()
-                A[P + r + k: P + r + k + r * l] = reg[:, :l].reshape(-1)
+                if fuse:
+                    # each leaf block's own gather rides along: its row roots
+                    # to the stash, its lp-key blocks to their vEB slots after
+                    # its first rp places, which the slide leaves as they were
+                    blocks = reg[:, :l]
+                    roots = blocks[:, :rp * (lp + 1)].reshape(r, rp, lp + 1)[:, :, lp]
+                    put(soff, roots.reshape(-1).copy())
+                    dst = A[P + r + k: P + r + k + r * l].reshape(r, l)
+                    dst[:, rp:] = _fuse_blocks(blocks, fuse, rp, lp)
+                else:
+                    A[P + r + k: P + r + k + r * l] = reg[:, :l].reshape(-1)
                 A[P + r * (l + 1) + k: P + r * (l + 1) + k + L] = tail
+        elif t == OP_PART_ROOTS:
+            # the rp roots of count blocks, stash <-> their vEB slots at the
+            # block fronts
+            base, stride, count, rp, soff, fuse, inv = p[0], p[1], p[2], p[3], p[4], p[5], p[6]
+            pos = lds_perm(LDS_MVEB if fuse == 1 else LDS_VEB, rp, 0, 0)
+            for i in range(count):
+                if inv:
+                    put(soff + i * rp, A[base + i * stride + pos].copy())
+                else:
+                    A[base + i * stride + pos] = stash[soff + i * rp: soff + (i + 1) * rp]
         else:
             raise ValueError(f"unknown op {t}")
     return A

@@ -59,6 +59,30 @@ def test_plans_replay_to_oracle(layout, bs, lds, row_stash, monkeypatch):
                 (layout, b, lds, n)
 
 
+@pytest.mark.parametrize("layout", ["mveb", "veb"])
+def test_fused_sparse_slide_plans_replay(layout):
+    """Trees whose top-level gather is a chained sparse slide carry their leaf
+    blocks' own gathers in it (perfect and non-perfect trees): the plans hold
+    such a fused slide, replay to the oracle's layout, their inverses replay
+    to sorted order, and forward then inverse leaves arbitrary data as it was"""
+    rng = np.random.default_rng(41)
+    for n in [300007, 1048575, 1048576, 3_000_017]:
+        buf = tl_amd.plan_export(layout, n, 0, "u64", 0)
+        ops, _ = pe.decode(buf)
+        assert any(int(o[0]) == pe.OP_CHAINED and int(o[20]) for o in ops), (layout, n)
+        A = np.arange(n, dtype=np.int64)
+        pe.replay(buf, A)
+        assert np.array_equal(A, ol.layout_ranks(layout, n, 0).astype(np.int64)), (layout, n)
+        inv = tl_amd.plan_export(layout, n, 0, "u64", 0, inverse=True)
+        pe.replay(inv, A)
+        assert np.array_equal(A, np.arange(n)), (layout, n)
+        X = rng.integers(-(1 << 40), 1 << 40, size=n)
+        Y = X.copy()
+        pe.replay(buf, Y)
+        pe.replay(inv, Y)
+        assert np.array_equal(X, Y), (layout, n)
+
+
 def test_full_size_plans_are_in_bounds():
     for layout, b, n in [("btree", 16, 1 << 30), ("btree", 16, 10 ** 8), ("bst", 1, 1 << 30),
                          ("bst", 1, 3 << 28), ("mveb", 0, (1 << 29) - 1),
@@ -98,7 +122,24 @@ class _Move:
     def __init__(self, op):
         (self.kind, self.P, self.L, self.k, self.r, self.l, self.CH,
          self.nchunk) = (int(v) for v in op[11:19])
+        self.fuse, self.rp, self.lp, self.soff = (int(v) for v in op[20:24])
         self.nmoves = int(op[3])
+
+    def fused(self, t):
+        """a fused leaf-block move of a sparse slide or spread"""
+        if not self.fuse:
+            return False
+        if self.kind == pe.MV_SPARSE_SPREAD:
+            return t < self.r * self.nchunk
+        return t >= (self.L + self.CH - 1) // self.CH
+
+
+def _wait_len(m, t, ln):
+    """the destination length a move waits on (a fused slide move writes its
+    leaf keys only)"""
+    if m.kind == pe.MV_SPARSE_SLIDE and m.fused(t):
+        return ln - ln // (m.lp + 1)
+    return ln
 
 
 def _written(m, dst, ln):
@@ -135,8 +176,9 @@ def test_chained_moves_wait_for_every_overwritten_source(inverse):
             srcs = sorted((s, s + ln, t) for t, (s, d, ln) in enumerate(moves))
             starts = [x[0] for x in srcs]
             for t, (s, d, ln) in enumerate(moves):
-                waited = set(bem.awaited_moves(m, t, d, ln))
-                for a, z in _written(m, d, ln):
+                wl = _wait_len(m, t, ln)
+                waited = set(bem.awaited_moves(m, t, d, wl))
+                for a, z in _written(m, d, wl):
                     i = max(0, bisect.bisect_right(starts, a) - 1)
                     while i < len(srcs) and srcs[i][0] < z:
                         s0, s1, u = srcs[i]
@@ -146,12 +188,33 @@ def test_chained_moves_wait_for_every_overwritten_source(inverse):
             # sequential replay in move order against the specified effect
             one = np.concatenate([buf[:2], op.astype(np.uint64)]).astype(np.uint64)
             one[0], one[1] = 1, 0
-            sep = -1 - np.arange(m.r, dtype=np.int64)   # stash contents for a row spread
+            # stash contents for a row spread or a fused spread's roots
+            sep = -1 - np.arange(max(m.r, m.soff + m.r * m.rp), dtype=np.int64)
             spec = np.arange(n, dtype=np.int64)
             pe.replay(one, spec, stash=sep.copy())
             seq = np.arange(n, dtype=np.int64)
-            for s, d, ln in moves:
+            pos = None
+            if m.fuse:
+                pos = pe.lds_perm(pe.LDS_MVEB if m.fuse == 1 else pe.LDS_VEB, m.lp, 0, 0)
+            for t, (s, d, ln) in enumerate(moves):
                 v = seq[s:s + ln].copy()
+                if m.fused(t):
+                    L1 = m.lp + 1
+                    idx = np.arange(ln)
+                    a, c = idx // L1, idx % L1
+                    if m.kind == pe.MV_SPARSE_SLIDE:   # leaves to their slots
+                        keep = c < m.lp
+                        seq[d + a[keep] * m.lp + pos[c[keep]]] = v[keep]
+                    else:   # rows back, the roots from the stash
+                        rel = d - m.P
+                        i = rel // (m.l + 1)
+                        a0 = (rel - i * (m.l + 1)) // L1
+                        leaf = c < m.lp
+                        out = np.empty(ln, dtype=np.int64)
+                        out[leaf] = seq[s + a[leaf] * m.lp + pos[c[leaf]]]
+                        out[~leaf] = sep[m.soff + i * m.rp + a0 + a[~leaf]]
+                        seq[d:d + ln] = out
+                    continue
                 if m.kind == pe.MV_ROW_SLIDE:
                     rows = v.reshape(-1, m.l + 1)[:, :m.l].reshape(-1)
                     seq[d:d + rows.size] = rows

@@ -657,10 +657,11 @@ static tl_status build_batch(Batch &B, Ctx *c) {
     Tuning tu_persistent;
     tu_persistent.lds_bytes = std::min<size_t>(B.cfg.lds_bytes, c->lds_max - LDS_STATIC_RESERVE);
     // the persistent executors rotate by reversals (they have no lane walk)
-    // and have no folded bit reversal
+    // and have no folded bit reversal or fused sparse slide
     tu_persistent.rotate_walk = false;
     tu_persistent.fuse_leaves = false;
     tu_persistent.bst_fold = false;
+    tu_persistent.fuse_slide = false;
     struct Pending {
         uint32_t op, depth, array;
     };
@@ -859,7 +860,10 @@ static tl_status build_queues(Batch &B) {
 
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
-    tu.bst_fold = false;   // the multi-array bit-reversal launch has no folded form
+    // the multi-array launches have no folded bit reversal and no fused
+    // sparse slide
+    tu.bst_fold = false;
+    tu.fuse_slide = false;
     if (B.cfg.tile_k == 5 || B.cfg.tile_k == 6) tu.bitrev_k = B.cfg.tile_k;
     if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;

@@ -290,6 +290,13 @@ __global__ __launch_bounds__(256) void k_level_heads(const T *__restrict__ A, T 
 //   MV_ROW_SLIDE: r rows of (l keys, 1 separator) at P become
 //     [r separators (left in the stash)][r*l keys]; a move carries CH whole
 //     rows, writes their keys contiguously and their separators to the stash.
+//   MV_SPARSE_SLIDE with fuse = 1 (mvEB) or 2 (classic vEB): every leaf
+//     block of l keys is itself a perfect tree of rp rows of (lp keys,
+//     1 root) and a last block of lp keys (lp + 1 a power of 2), and its own
+//     gather rides along: a leaf-block move carries whole rows (CH a multiple
+//     of lp + 1), writes each row's lp keys in their block's vEB order to
+//     the block's final place (after its rp roots) and the row's root to
+//     stash[soff + i*rp + row]; OP_PART_ROOTS lays the roots out afterwards.
 //   MV_ROW_SPREAD, MV_SPARSE_SPREAD: the inverses (layout -> sorted order).
 //     The keys move left, back into rows of l keys with one gap after each
 //     row; moves are numbered from the left, so every source a move
@@ -317,7 +324,26 @@ struct MoveSpec {
     uint64_t r, l;    // slides: rows and leaf-block length
     uint64_t CH;      // chunk length (rows per move for the row slide)
     uint64_t nchunk;  // chunks per region / per leaf block, or row-slide moves
+    uint32_t fuse;    // sparse slide / spread: leaf blocks' own gather (1 mvEB, 2 vEB)
+    uint64_t rp, lp;  // ... its rows and leaf-block length
+    uint64_t soff;    // ... stash slot of block 0's first row root
 };
+
+// a fused leaf-block move of a sparse slide (t counted from 0 in move order)
+__device__ inline bool fused_slide_move(const MoveSpec &m, uint64_t t) {
+    return m.fuse && t >= (m.L + m.CH - 1) / m.CH;
+}
+
+// ... of a sparse spread (leaf blocks first)
+__device__ inline bool fused_spread_move(const MoveSpec &m, uint64_t t) {
+    return m.fuse && t < m.r * m.nchunk;
+}
+
+// leaf keys of a fused move of len source keys: its full rows each hold one
+// root, the leaf block's last lp keys (a short last row) none
+__device__ inline uint64_t fused_leaves(const MoveSpec &m, uint64_t len) {
+    return len - len / (m.lp + 1);
+}
 
 __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uint64_t &dst,
                                uint64_t &len) {
@@ -351,7 +377,8 @@ __device__ inline void move_at(const MoveSpec &m, uint64_t t, uint64_t &src, uin
         const uint64_t c = m.nchunk - 1 - t % m.nchunk;
         src = m.P + i * (m.l + 1) + c * m.CH;
         len = min(m.CH, m.l - c * m.CH);
-        dst = m.P + m.r + m.k + i * m.l + c * m.CH;
+        dst = m.P + m.r + m.k + i * m.l +
+              (m.fuse ? m.rp + c * m.CH / (m.lp + 1) * m.lp : c * m.CH);
     }
 }
 
@@ -371,8 +398,10 @@ __device__ inline void move_at_spread(const MoveSpec &m, uint64_t t, uint64_t &s
     // leaf blocks first (block 0, chunk 0 first), then the tail chunks
     const uint64_t nleaf = m.r * m.nchunk;
     if (t < nleaf) {
+        // len counts destination keys (a fused move's roots included)
         const uint64_t i = t / m.nchunk, c = t - i * m.nchunk;
-        src = m.P + m.r + m.k + i * m.l + c * m.CH;
+        src = m.P + m.r + m.k + i * m.l +
+              (m.fuse ? m.rp + c * m.CH / (m.lp + 1) * m.lp : c * m.CH);
         len = min(m.CH, m.l - c * m.CH);
         dst = m.P + i * (m.l + 1) + c * m.CH;
         return;
@@ -501,10 +530,14 @@ __device__ inline bool wait_deps_spread(const MoveSpec &m, uint64_t t, uint64_t 
         uhi = z / (m.CH * m.l);
     } else {
         const uint64_t leaf = m.r * m.l;
+        // the move whose source holds x; fused: a block's roots (its first rp
+        // keys, read by OP_PART_ROOTS before) count as its first move's
         auto num = [&](uint64_t x) {
             if (x < leaf) {
-                const uint64_t i = x / m.l;
-                return i * m.nchunk + (x - i * m.l) / m.CH;
+                const uint64_t i = x / m.l, w = x - i * m.l;
+                if (!m.fuse) return i * m.nchunk + w / m.CH;
+                const uint64_t rows = w < m.rp ? 0 : (w - m.rp) / m.lp;
+                return i * m.nchunk + rows / (m.CH / (m.lp + 1));
             }
             return m.r * m.nchunk + (x - leaf) / m.CH;
         };
@@ -551,9 +584,48 @@ __device__ inline void load_row_spread(const T *A, const T *stash, const MoveSpe
 // spread
 enum : int { CH_FORWARD = 0, CH_ROW_SPREAD = 1, CH_SPARSE_SPREAD = 2 };
 
+// loads of a fused sparse-spread move, indexed by the destination (rows of
+// lp keys and a root): key c of row a comes from its block's vEB slot
+// src + a*lp + slot[c], the root from the stash
+template <typename T, int PER>
+__device__ inline void load_fused_spread(const T *A, const T *stash, const MoveSpec &ms,
+                                         uint64_t src, uint64_t dst, uint64_t len,
+                                         const uint32_t *slot, T (&v)[PER]) {
+    const uint32_t lg = 63u - (uint32_t)__clzll(ms.lp + 1), lp = (uint32_t)ms.lp;
+    const uint64_t rel = dst - ms.P, i = rel / (ms.l + 1);
+    const T *st = stash + ms.soff + i * ms.rp + ((rel - i * (ms.l + 1)) >> lg);
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t idx = threadIdx.x + (uint32_t)j * 256u;
+        if (idx < len) {
+            const uint32_t a = idx >> lg, c = idx & (lp);
+            v[j] = c == lp ? st[a] : A[src + (uint64_t)a * lp + slot[c]];
+        }
+    }
+}
+
 template <typename T, int PER, int MODE = CH_FORWARD>
 __device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t src, uint64_t dst,
-                                  uint64_t len, const T (&v)[PER]) {
+                                  uint64_t len, const T (&v)[PER], bool fused = false,
+                                  const uint32_t *slot = nullptr) {
+    if (MODE == CH_FORWARD && fused) {
+        // rows of (lp keys, 1 root): keys to their vEB slots, roots to the stash
+        const uint32_t lg = 63u - (uint32_t)__clzll(ms.lp + 1), lp = (uint32_t)ms.lp;
+        const uint64_t rel = src - ms.P, i = rel / (ms.l + 1);
+        T *st = stash + ms.soff + i * ms.rp + ((rel - i * (ms.l + 1)) >> lg);
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint32_t idx = threadIdx.x + (uint32_t)j * blockDim.x;
+            if (idx < len) {
+                const uint32_t a = idx >> lg, c = idx & lp;
+                if (c == lp)
+                    st[a] = v[j];
+                else
+                    A[dst + (uint64_t)a * lp + slot[c]] = v[j];
+            }
+        }
+        return;
+    }
     if constexpr (MODE == CH_ROW_SPREAD) {
         const uint64_t tot = len / ms.l * (ms.l + 1);
 #pragma unroll
@@ -586,19 +658,26 @@ __device__ inline void store_move(T *A, T *stash, const MoveSpec &ms, uint64_t s
     }
 }
 
-// one chained move, t, by the whole workgroup: load, publish, wait, store
+// one chained move, t, by the whole workgroup: load, publish, wait, store.
+// slot: the vEB slots of a fused move's lp-key blocks (LDS)
 template <typename T, int PER, int MODE>
 __device__ inline void chained_move(T *__restrict__ A, const MoveSpec &ms, uint64_t t,
-                                    uint32_t *flags, uint32_t *err, uint64_t limit, T *stash) {
+                                    uint32_t *flags, uint32_t *err, uint64_t limit, T *stash,
+                                    const uint32_t *slot = nullptr) {
     {
         uint64_t src, dst, len;
         if constexpr (MODE != CH_FORWARD)
             move_at_spread(ms, t, src, dst, len);
         else
             move_at(ms, t, src, dst, len);
+        const bool fused = MODE == CH_FORWARD     ? fused_slide_move(ms, t)
+                           : MODE == CH_SPARSE_SPREAD ? fused_spread_move(ms, t)
+                                                      : false;
         T v[PER];
         if constexpr (MODE == CH_ROW_SPREAD) {
             load_row_spread<T, PER>(A, stash, ms, src, len, v);
+        } else if (MODE == CH_SPARSE_SPREAD && fused) {
+            load_fused_spread<T, PER>(A, stash, ms, src, dst, len, slot, v);
         } else {
 #pragma unroll
             for (int j = 0; j < PER; j++) {
@@ -615,29 +694,42 @@ __device__ inline void chained_move(T *__restrict__ A, const MoveSpec &ms, uint6
             bool ok;
             if constexpr (MODE != CH_FORWARD)
                 ok = wait_deps_spread(ms, t, dst, len, flags, err, limit);
-            else
-                ok = wait_deps(ms, t, dst, len, flags, err, limit);
+            else   // a fused move writes its leaf keys only
+                ok = wait_deps(ms, t, dst, fused ? fused_leaves(ms, len) : len, flags, err, limit);
             s_ok = ok ? 1u : 0u;
         }
         __syncthreads();
         // a move whose wait gave up (the call then reports TL_ETIMEOUT) keeps
         // its loads to itself: its destination may hold keys not yet read
-        if (s_ok) store_move<T, PER, MODE>(A, stash, ms, src, dst, len, v);
+        if (s_ok) store_move<T, PER, MODE>(A, stash, ms, src, dst, len, v, fused, slot);
     }
 }
+
+// vEB slots of an lp-key block in LDS, for the fused sparse moves
+__device__ inline void fill_fused_slots(const MoveSpec &ms, uint32_t *slot) {
+    if (!ms.fuse) return;
+    for (uint32_t x = threadIdx.x; x < (uint32_t)ms.lp; x += blockDim.x)
+        slot[x] = mveb_pos32(x, (uint32_t)ms.lp, ms.fuse == 2);
+    __syncthreads();
+}
+
+// largest lp of a fused sparse move (its slot table in LDS)
+constexpr uint64_t FUSED_LP_MAX = 2047;
 
 template <typename T, int PER, int MODE = CH_FORWARD>
 __global__ __launch_bounds__(256) void k_chained(T *__restrict__ A, MoveSpec ms, uint64_t nmoves,
                                                  uint32_t *flags, unsigned long long *counter,
                                                  uint32_t *err, uint64_t limit, T *stash) {
     __shared__ uint64_t s_t;
+    __shared__ uint32_t s_slot[FUSED_LP_MAX];
+    if constexpr (MODE != CH_ROW_SPREAD) fill_fused_slots(ms, s_slot);
     for (;;) {
         if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
         __syncthreads();
         const uint64_t t = s_t;
         __syncthreads();
         if (t >= nmoves) return;
-        chained_move<T, PER, MODE>(A, ms, t, flags, err, limit, stash);
+        chained_move<T, PER, MODE>(A, ms, t, flags, err, limit, stash, s_slot);
     }
 }
 
@@ -690,6 +782,28 @@ __global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t
                                                      uint64_t w, const T *__restrict__ stash) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < w) walk_lane<T, K>(A, L, c, w, stash, j);
+}
+
+// ---------------------------------------------------------------------------
+// the rp roots of `count` blocks (block i at base + i*stride) from the stash
+// (stash[soff + i*rp + a], put there by a fused sparse slide) to their vEB
+// slots at the block's front; inv: from there back to the stash
+template <typename T>
+__global__ __launch_bounds__(256) void k_part_roots(T *__restrict__ A, T *__restrict__ stash,
+                                                    uint64_t base, uint64_t stride, uint64_t count,
+                                                    uint32_t rp, uint64_t soff, uint32_t classic,
+                                                    uint32_t inv) {
+    const uint64_t total = count * rp;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = g / rp;
+        const uint32_t a = (uint32_t)(g - i * rp);
+        T *slot = A + base + i * stride + mveb_pos32(a, rp, classic != 0);
+        if (inv)
+            stash[soff + g] = *slot;
+        else
+            *slot = stash[soff + g];
+    }
 }
 
 // ---------------------------------------------------------------------------

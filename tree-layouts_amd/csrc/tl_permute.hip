@@ -559,6 +559,16 @@ static tl_status launch_plan(Ctx *c, const Plan &pl, T *A, hipStream_t s, const 
                 TL_TRY((levels_launch<T, 0>(c, A, t0, d, K, per, nullptr, s)));
             break;
         }
+        case OP_PART_ROOTS: {
+            const uint64_t total = op.p[2] * op.p[3];
+            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, c->num_cus * 64ull);
+            if (total)
+                hipLaunchKernelGGL(k_part_roots<T>, dim3(g), dim3(256), 0, s, A, stash, op.p[0],
+                                   op.p[1], op.p[2], (uint32_t)op.p[3], op.p[4],
+                                   (uint32_t)(op.p[5] == 2), (uint32_t)op.p[6]);
+            TL_LAUNCH_CHECK();
+            break;
+        }
         case OP_HEADS: {
             const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
             const uint64_t total = rowend_slots(d, K, t0);
@@ -1029,7 +1039,7 @@ static tl_status permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_d
 
 // Serialized plan: per op 32 words
 //   [type, nruns, nitems, p0..p7, ms.kind, ms.P, ms.L, ms.k, ms.r, ms.l, ms.CH, ms.nchunk,
-//    runs_at, 0...] followed by the blob.
+//    runs_at, ms.fuse, ms.rp, ms.lp, ms.soff, 0...] followed by the blob.
 static tl_status export_plan(Layout lay, uint64_t n, uint64_t b, tl_dtype dt, uint64_t lds_bytes,
                              uint64_t *buf, uint64_t cap, uint64_t *used, bool inverse) {
     uint32_t W;
@@ -1069,6 +1079,10 @@ static tl_status export_plan(Layout lay, uint64_t n, uint64_t b, tl_dtype dt, ui
         w[17] = op.ms.CH;
         w[18] = op.ms.nchunk;
         w[19] = op.runs_at;
+        w[20] = op.ms.fuse;
+        w[21] = op.ms.rp;
+        w[22] = op.ms.lp;
+        w[23] = op.ms.soff;
         for (int i = 0; i < 32; i++) *o++ = w[i];
     }
     for (uint64_t v : pl.blob) *o++ = v;

@@ -50,7 +50,9 @@ enum OpType {
     OP_CHAINED,
     OP_BITREV_LEVELS,  // the per-level reversals of levels p[0] .. p[1]-1, tiles 2^p[2]
     OP_ROTATE,         // [p0, p0+p1) rotated right by p2 in one pass, p3 lanes (stash)
-    OP_HEADS           // the heads of levels p[0] .. p[1]-1 (tiles 2^p[2]) to the stash
+    OP_HEADS,          // the heads of levels p[0] .. p[1]-1 (tiles 2^p[2]) to the stash
+    OP_PART_ROOTS      // the p[3] roots of p[2] blocks at p[0] + i*p[1] from stash[p[4] ..]
+                       // to their vEB slots (p[5]: 1 mvEB, 2 classic; p[6]: inverse)
 };
 
 // A BST of 2^D keys has one leaf, the smallest key, which goes last; every
@@ -196,7 +198,8 @@ struct Tuning {
     bool fuse_leaves = true;               // vEB leaf layouts fused into streamed gathers
     uint64_t walk_lanes = 1ull << 24;      // lanes (stashed keys) of a one-pass rotation
     int64_t walk_env = -1;                 // $TL_WALK_LANES when set (0: no one-pass rotations)
-    bool bst_fold = true;                  // BST of 2^D keys: leaf move folded into phase one
+    bool bst_fold = true;                  // BST of 2^D keys: kept one place right
+    bool fuse_slide = true;                // vEB: leaf blocks' gathers ride in the sparse slide
 };
 
 inline void tuning_env(Tuning &t) {
@@ -208,6 +211,8 @@ inline void tuning_env(Tuning &t) {
     if (w) t.walk_env = atoll(w);
     const char *f = getenv("TL_BST_FOLD");
     if (f && atoi(f) == 0) t.bst_fold = false;
+    const char *fs = getenv("TL_FUSE_SLIDE");
+    if (fs && atoi(fs) == 0) t.fuse_slide = false;
 }
 
 inline uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -505,7 +510,12 @@ struct Planner {
     // first, then the extra roots), leaf blocks and tail slide right by `extra`
     // more than a plain gather, and the stash comes back as the first r+extra
     // keys.  With tail = extra = 0 this is the sparse gather.
-    void sparse_slide(uint64_t P, uint64_t r, uint64_t l, uint64_t tail, uint64_t extra) {
+    // fuse (1 mvEB, 2 classic vEB): every leaf block is a perfect tree of rp
+    // rows of (lp keys, 1 root) and a last block of lp keys, whose own gather
+    // and whose lp-key blocks' layouts ride in the same moves; its roots go
+    // to the stash behind the slide's own and are laid out by OP_PART_ROOTS.
+    void sparse_slide(uint64_t P, uint64_t r, uint64_t l, uint64_t tail, uint64_t extra,
+                      uint32_t fuse = 0, uint64_t rp = 0, uint64_t lp = 0) {
         copy(P + l, l + 1, r, true);                               // row roots
         if (extra) copy(P + r * (l + 1) + tail, 1, extra, true, r);   // trailing roots
         Op op;
@@ -517,11 +527,30 @@ struct Planner {
         op.ms.L = tail;
         op.ms.k = extra;
         op.ms.CH = std::min<uint64_t>(l, CHAINED_KEYS);
+        if (fuse) {   // whole rows per move
+            op.ms.fuse = fuse;
+            op.ms.rp = rp;
+            op.ms.lp = lp;
+            op.ms.soff = r + extra;
+            op.ms.CH = std::min<uint64_t>(CHAINED_KEYS / (lp + 1), rp) * (lp + 1);
+            pl.max_stash = std::max(pl.max_stash, op.ms.soff + r * rp);
+        }
         op.ms.nchunk = (l + op.ms.CH - 1) / op.ms.CH;
         op.p[0] = (tail + op.ms.CH - 1) / op.ms.CH + r * op.ms.nchunk;
         pl.max_moves = std::max(pl.max_moves, op.p[0]);
         pl.ops.push_back(op);
         copy(P, 1, r + extra, false);   // roots to the front
+        if (fuse) {
+            Op pr;
+            pr.type = OP_PART_ROOTS;
+            pr.p[0] = P + r + extra;
+            pr.p[1] = l;
+            pr.p[2] = r;
+            pr.p[3] = rp;
+            pr.p[4] = r + extra;
+            pr.p[5] = fuse;
+            pl.ops.push_back(pr);
+        }
     }
 };
 
@@ -651,30 +680,55 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
     veb_split(d, rd, ld, classic);
     const uint64_t r = (1ull << rd) - 1, l = (1ull << ld) - 1;
     std::vector<VebBatch> kids;
+    // a chained gather of whole leaf blocks (perfect trees of depth ld) can
+    // carry their own gathers and lp-key layouts (fused sparse slide)
+    uint32_t rd2 = 0, ld2 = 0;
+    veb_split(ld, rd2, ld2, classic);
+    const uint64_t rp = (1ull << rd2) - 1, lp = (1ull << ld2) - 1;
+    auto fusable = [&](uint64_t rows) {
+        return Q.tu.fuse_slide && ld >= 2 && lp <= FUSED_LP_MAX && lp + 1 <= CHAINED_KEYS &&
+               rp <= 65535 && r + rows * rp <= Q.tu.row_stash_max;
+    };
     if (n == (1ull << d) - 1) {
-        if (Q.gather_sparse(B.runs, r, l, classic ? 2u : 1u)) {
-            // both kids were laid out inside the gather
-            sub.blob.swap(P.pl.blob);
-            P.pl.max_stash = std::max(P.pl.max_stash, sub.max_stash);
-            P.pl.max_moves = std::max(P.pl.max_moves, sub.max_moves);
-            for (auto &op : sub.ops) depth_ops[depth].push_back(op);
-            return;
-        }
-        bool contiguous = (rd == ld);
-        for (auto &x : B.runs)
-            if (x.count > 1 && x.stride != n) contiguous = false;
-        if (contiguous) {
-            VebBatch c{l, {}};
-            for (auto &x : B.runs) c.runs.push_back({x.start, l, x.count * (n / l)});
-            kids.push_back(c);
-        } else {
-            VebBatch roots{r, B.runs};
-            VebBatch lv{l, {}};
+        uint64_t items = 0;
+        for (auto &x : B.runs) items += x.count;
+        const bool streamed = r * Q.W <= 65536 && (items >= 64 || r * (l + 1) <= 131072);
+        if (!streamed && r <= Q.tu.row_stash_max && fusable(r)) {
+            // the rows' leaf blocks are done; the last leaf block stays
+            // where it is and is laid out on its own
+            VebBatch last{l, {}};
             for (auto &x : B.runs)
-                for (uint64_t i = 0; i < x.count; i++)
-                    lv.runs.push_back({x.start + i * x.stride + r, l, r + 1});
-            kids.push_back(roots);
-            kids.push_back(lv);
+                for (uint64_t i = 0; i < x.count; i++) {
+                    Q.sparse_slide(x.start + i * x.stride, r, l, 0, 0, classic ? 2u : 1u, rp, lp);
+                    last.runs.push_back({x.start + i * x.stride + r * (l + 1), 0, 1});
+                }
+            kids.push_back({r, B.runs});
+            kids.push_back(last);
+        } else {
+            if (Q.gather_sparse(B.runs, r, l, classic ? 2u : 1u)) {
+                // both kids were laid out inside the gather
+                sub.blob.swap(P.pl.blob);
+                P.pl.max_stash = std::max(P.pl.max_stash, sub.max_stash);
+                P.pl.max_moves = std::max(P.pl.max_moves, sub.max_moves);
+                for (auto &op : sub.ops) depth_ops[depth].push_back(op);
+                return;
+            }
+            bool contiguous = (rd == ld);
+            for (auto &x : B.runs)
+                if (x.count > 1 && x.stride != n) contiguous = false;
+            if (contiguous) {
+                VebBatch c{l, {}};
+                for (auto &x : B.runs) c.runs.push_back({x.start, l, x.count * (n / l)});
+                kids.push_back(c);
+            } else {
+                VebBatch roots{r, B.runs};
+                VebBatch lv{l, {}};
+                for (auto &x : B.runs)
+                    for (uint64_t i = 0; i < x.count; i++)
+                        lv.runs.push_back({x.start + i * x.stride + r, l, r + 1});
+                kids.push_back(roots);
+                kids.push_back(lv);
+            }
         }
     } else {
         for (auto &x : B.runs)
@@ -685,14 +739,20 @@ inline void plan_mveb_rec(Planner &P, std::vector<std::vector<Op>> &depth_ops,
                 // incomplete one and empty ones) follow the incomplete block;
                 // when the gather is a chained pass they ride along in it
                 const bool chained = nf && !(nf * Q.W <= 65536 && nf * (l + 1) <= 131072);
+                // the full leaf blocks' own gathers ride in the slide
+                const bool fuse = chained && fusable(nf);
                 if (chained && r <= Q.tu.row_stash_max) {
-                    Q.sparse_slide(off, nf, l, inc, r - nf);
+                    if (fuse)
+                        Q.sparse_slide(off, nf, l, inc, r - nf, classic ? 2u : 1u, rp, lp);
+                    else
+                        Q.sparse_slide(off, nf, l, inc, r - nf);
                 } else {
                     Q.gather_sparse({{off, 0, 1}}, nf, l);
                     if (nf < r) Q.rotate_right({{off + nf, 0, 1}}, n - nf, r - nf);
                 }
                 kids.push_back({r, {{off, 0, 1}}});
-                if (nf) kids.push_back({l, {{off + r, l, nf}}});
+                if (nf && !(fuse && chained && r <= Q.tu.row_stash_max))
+                    kids.push_back({l, {{off + r, l, nf}}});
                 if (inc) kids.push_back({inc, {{off + n - inc, 0, 1}}});
             }
     }
@@ -733,6 +793,14 @@ inline uint64_t runs_end(const Plan &pl, const Op &op, uint64_t item_len) {
     return end;
 }
 
+// a fused sparse slide / spread: leaf blocks of rp rows of (lp keys, 1 root)
+// and lp keys, lp + 1 a power of 2 dividing the chunk, roots in the stash
+inline bool fused_spec_ok(const Plan &pl, const MoveSpec &m) {
+    const uint64_t L1 = m.lp + 1;
+    return m.fuse <= 2 && m.lp >= 1 && m.lp <= FUSED_LP_MAX && (L1 & m.lp) == 0 &&
+           m.CH % L1 == 0 && m.l == m.rp * L1 + m.lp && m.soff + m.r * m.rp <= pl.max_stash;
+}
+
 inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
     for (size_t k = 0; k < pl.ops.size(); k++) {
         const Op &op = pl.ops[k];
@@ -767,6 +835,12 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
             }
             break;
         }
+        case OP_PART_ROOTS:
+            end = op.p[2] ? op.p[0] + (op.p[2] - 1) * op.p[1] + op.p[3] : 0;
+            if (op.p[3] > op.p[1] || op.p[3] > 65535 || op.p[5] < 1 || op.p[5] > 2 ||
+                op.p[4] + op.p[2] * op.p[3] > pl.max_stash)
+                neg = true;
+            break;
         case OP_HEADS: {
             const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1], K = (uint32_t)op.p[2];
             if (t0 != 2 * K || d <= t0 || d > 62 || rowend_slots(d, K, t0) > pl.max_stash)
@@ -803,11 +877,13 @@ inline bool validate_plan(const Plan &pl, uint64_t N, std::string &why) {
                 if (m.CH == 0 || m.CH > CHAINED_KEYS || m.CH > m.l ||
                     op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
                     neg = true;
+                if (m.fuse && !fused_spec_ok(pl, m)) neg = true;
             } else if (m.kind == MV_SPARSE_SPREAD) {
                 end = m.P + m.r * (m.l + 1) + m.L + m.k;
                 if (m.CH == 0 || m.CH > CHAINED_KEYS || m.CH > m.l || m.nchunk * m.CH < m.l ||
                     op.p[0] != (m.L + m.CH - 1) / m.CH + m.r * m.nchunk)
                     neg = true;
+                if (m.fuse && !fused_spec_ok(pl, m)) neg = true;
                 break;
             } else if (m.kind == MV_ROW_SLIDE || m.kind == MV_ROW_SPREAD) {
                 end = m.P + m.r * (m.l + 1);
@@ -884,6 +960,7 @@ inline void invert_plan(const Plan &f, Plan &g) {
             if (op.p[3] != BITREV_PLAIN) op.p[3] = BITREV_SHIFT + BITREV_UNFOLD - op.p[3];
             break;
         case OP_HEADS: continue;   // the unfold keeps row ends instead
+        case OP_PART_ROOTS: op.p[6] ^= 1; break;
         case OP_ROTATE: op.p[2] = op.p[1] - op.p[2]; break;
         case OP_CHAINED: {
             MoveSpec &m = op.ms;
