@@ -62,6 +62,17 @@ def test_permute_stash_budgets(layout, b, row_stash, dev, monkeypatch):
         assert np.array_equal(got, exp), (layout, b, row_stash, n)
 
 
+@pytest.mark.parametrize("layout,b", [("btree", 16), ("btree", 8), ("bst", 1)])
+@pytest.mark.parametrize("kind", ["u32", "u64"])
+def test_rotation_walk_block(layout, b, kind, dev, monkeypatch):
+    """Rotation lane walks in workgroups of 256 lanes (the default is 1024)
+    give the same layouts"""
+    monkeypatch.setenv("TL_WALK_BLOCK", "256")
+    for n in MEDIUM + [3_000_017, 17 ** 6 - 1 + 12345]:
+        got, exp = _run(layout, b, kind, n, dev)
+        assert np.array_equal(got, exp), (layout, b, kind, n)
+
+
 @pytest.mark.parametrize("tile_k", ["5", "6"])
 @pytest.mark.parametrize("kind", ["u32", "u64"])
 def test_bst_bitrev_tile_sizes(tile_k, kind, dev, monkeypatch):

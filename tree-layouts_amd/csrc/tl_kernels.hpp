@@ -777,9 +777,9 @@ __device__ inline void walk_lane(T *__restrict__ A, uint64_t L, uint64_t c, uint
     }
 }
 
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k_rotate_walk(T *__restrict__ A, uint64_t L, uint64_t c,
-                                                     uint64_t w, const T *__restrict__ stash) {
+template <typename T, int K, int BS = 256>
+__global__ __launch_bounds__(BS) void k_rotate_walk(T *__restrict__ A, uint64_t L, uint64_t c,
+                                                    uint64_t w, const T *__restrict__ stash) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < w) walk_lane<T, K>(A, L, c, w, stash, j);
 }
