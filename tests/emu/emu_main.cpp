@@ -55,13 +55,13 @@ int main(int argc, char **argv) {
             for (uint64_t j = 0; j < nhit; j++) a.q32.push_back(a.keys32[pick(rng)]);
             for (uint64_t j = 0; j < nmiss; j++) a.q32.push_back((int32_t)((int64_t)any(rng) - mid));
             a.A = malloc(a.n * 4); memcpy(a.A, a.keys32.data(), a.n * 4);
-            a.Q = malloc(a.nq * 4); memcpy(a.Q, a.q32.data(), a.nq * 4);
+            a.Q = malloc(a.nq * 4 + 4); if (a.nq) memcpy(a.Q, a.q32.data(), a.nq * 4);
         } else {
             a.keys64 = keys;
             for (uint64_t j = 0; j < nhit; j++) a.q64.push_back(keys[pick(rng)]);
             for (uint64_t j = 0; j < nmiss; j++) a.q64.push_back(any(rng));
             a.A = malloc(a.n * 8); memcpy(a.A, keys.data(), a.n * 8);
-            a.Q = malloc(a.nq * 8); memcpy(a.Q, a.q64.data(), a.nq * 8);
+            a.Q = malloc(a.nq * 8 + 8); if (a.nq) memcpy(a.Q, a.q64.data(), a.nq * 8);
         }
         a.ans = (uint64_t *)malloc(a.nq * 8);
         for (uint64_t j = 0; j < a.nq; j++) a.ans[j] = ~0ull;
@@ -104,7 +104,7 @@ int main(int argc, char **argv) {
             else if (a.layout == 1) or_search_btree_u64(e.data(), a.n, 8, a.q64.data(), a.nq, exp_ans.data(), 1);
             else or_search_mveb_u64(e.data(), a.n, a.q64.data(), a.nq, exp_ans.data(), 1);
         }
-        ans_ok = memcmp(exp_ans.data(), a.ans, a.nq * 8) == 0;
+        ans_ok = a.nq == 0 || memcmp(exp_ans.data(), a.ans, a.nq * 8) == 0;
         if (!lay_ok || !ans_ok) {
             if (bad < 20) fprintf(stderr, "[emu] array %d layout %d %s n=%llu: layout %s answers %s\n", i, a.layout,
                                   a.i32 ? "i32" : "u64", (unsigned long long)a.n, lay_ok ? "ok" : "BAD", ans_ok ? "ok" : "BAD");

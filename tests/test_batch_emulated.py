@@ -45,3 +45,37 @@ def test_persistent_executors_mixed_widths(emulator):
 @pytest.mark.parametrize("lds", [0, 32768])
 def test_persistent_executors_240_array_batch(emulator, lds):
     _run(emulator, 240, 18, lds, 4096, 5)
+
+
+@pytest.fixture(scope="module")
+def planned():
+    subprocess.run(["make", "-C", EMU, "-j4", "_build/emu_planned"], check=True,
+                   capture_output=True)
+    return os.path.join(EMU, "_build", "emu_planned")
+
+
+def _run_planned(planned, count, log2max, seed, one=None):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    if one:
+        env["EMU_ONE"] = one
+    r = subprocess.run([planned, str(count), str(log2max), "0", "0", str(seed)],
+                       capture_output=True, text=True, timeout=900, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert f"0 of {count} arrays differ from the oracle" in r.stderr, r.stderr[-4000:]
+
+
+def test_planned_path_mixed_batch(planned):
+    """The planned path (tl_batch_run's multi-array launches and batched
+    in-LDS launches, built with the emulated tl_permute.hip) on a mixed batch
+    of 40 arrays of up to 2^19 keys, permutations only: every array equals
+    the oracle's layout"""
+    _run_planned(planned, 40, 19, 7)
+
+
+@pytest.mark.parametrize("one", ["2 u64 300007", "2 i32 1048576", "2 u64 1048575"])
+def test_planned_path_fused_slides(planned, one):
+    """mvEB arrays whose top-level gather is a fused sparse slide (the leaf
+    blocks' own gathers in the multi-array chained moves, their roots laid
+    out by the multi-array part-roots launch)"""
+    _run_planned(planned, 1, 19, 7, one)
+

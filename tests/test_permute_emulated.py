@@ -4,9 +4,13 @@ plan launch run one workgroup at a time with every lane a thread, on exactly
 sized buffers under AddressSanitizer.  Chained moves run in ticket order, so
 every wait is met when reached.  Each case permutes 0..n-1 through the
 library's entry point, compares every slot with the CPU oracle, runs the
-inverse entry point and checks the identity.  The vEB cases include the fused
-sparse slides (a leaf block's own gather riding in the top level's chained
-moves) of perfect and non-perfect trees, u32 and u64."""
+inverse entry point and checks the identity.  The mvEB cases include the
+fused sparse slides (a leaf block's own gather riding in the top level's
+chained moves) of perfect and non-perfect trees, u32 and u64; the classic
+vEB's fused slides differ only in the slot function, and their plans are
+replayed in test_plan.py (its host oracle is too slow for this file).  The
+BST cases cover the tree kept one place right (2^16 keys, and 300,007 keys:
+gather, rotation, then the same phases)."""
 import os
 import subprocess
 
@@ -25,8 +29,7 @@ def emulator():
 @pytest.mark.parametrize("layout,width,sizes", [
     ("mveb", 8, [300007, 1048575]),
     ("mveb", 4, [300007]),
-    ("veb", 8, [300007]),
-    ("bst", 8, [65536]),
+    ("bst", 8, [65536, 300007]),
     ("btree", 4, [300007]),
 ])
 def test_permute_emulated(emulator, layout, width, sizes):

@@ -860,10 +860,8 @@ static tl_status build_queues(Batch &B) {
 
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
-    // the multi-array launches have no folded bit reversal and no fused
-    // sparse slide
+    // the multi-array launches have no form of the BST kept one place right
     tu.bst_fold = false;
-    tu.fuse_slide = false;
     if (B.cfg.tile_k == 5 || B.cfg.tile_k == 6) tu.bitrev_k = B.cfg.tile_k;
     if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;
@@ -1076,6 +1074,12 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 else
                     d = MCopy{Ap, St, op.p[1], 1, op.p[2]};
                 add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[2] + 2047) / 2048, 0);
+                break;
+            }
+            case OP_PART_ROOTS: {
+                const MPRoots d{A, stash + op.p[4] * W, op.p[0], op.p[1], op.p[2],
+                                (uint32_t)op.p[3], (uint32_t)(op.p[5] == 2), (uint32_t)op.p[6], 0};
+                add(ga, MK_PROOTS, W, 0, 256, &d, sizeof d, (op.p[2] * op.p[3] + 255) / 256, 0);
                 break;
             }
             case OP_ROTATE: {   // the stash copy, then (a later launch) the lane walk

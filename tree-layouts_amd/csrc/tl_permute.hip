@@ -821,6 +821,8 @@ __global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict_
                                                        uint64_t total, unsigned long long *counter,
                                                        uint32_t *err, uint64_t limit) {
     __shared__ uint64_t s_t;
+    __shared__ uint32_t s_slot[FUSED_LP_MAX];
+    uint64_t have = 0;   // (lp, fuse) of the slot table in s_slot; the same in every lane
     for (;;) {
         if (threadIdx.x == 0) s_t = atomicAdd(counter, 1ull);
         __syncthreads();
@@ -829,9 +831,38 @@ __global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict_
         if (t >= total) return;
         const uint32_t i = find_desc(pre, nd, t);
         const MChain m = d[i];
+        if (MODE != CH_ROW_SPREAD && m.ms.fuse) {
+            // a fused sparse move: the vEB slots of its array's lp-key blocks
+            // (every lane passed the barriers above since the last table use)
+            const uint64_t want = (m.ms.lp << 2) | m.ms.fuse;
+            if (want != have) {
+                fill_fused_slots(m.ms, s_slot);
+                have = want;
+            }
+        }
         chained_move<T, PER, MODE>((T *)m.A, m.ms, t - pre[i], (uint32_t *)m.flags, err, limit,
-                                   (T *)m.stash);
+                                   (T *)m.stash, s_slot);
     }
+}
+
+// OP_PART_ROOTS of many arrays: 256 roots per workgroup
+template <typename T>
+__global__ __launch_bounds__(256) void k_multi_proots(const MPRoots *__restrict__ d,
+                                                      const uint64_t *__restrict__ pre,
+                                                      uint32_t nd) {
+    const uint64_t t = blockIdx.x;
+    const uint32_t i = find_desc(pre, nd, t);
+    const MPRoots m = d[i];
+    const uint64_t g = (t - pre[i]) * 256 + threadIdx.x;
+    if (g >= m.count * m.rp) return;
+    const uint64_t blk = g / m.rp;
+    const uint32_t a = (uint32_t)(g - blk * m.rp);
+    T *slot = (T *)m.A + m.base + blk * m.stride + mveb_pos32(a, m.rp, m.classic != 0);
+    T *st = (T *)m.stash + g;
+    if (m.inv)
+        *st = *slot;
+    else
+        *slot = *st;
 }
 
 template <typename T, int K>
@@ -927,6 +958,10 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     case MK_WALK:
         hipLaunchKernelGGL(k_multi_walk<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
                            (const MWalk *)m.desc, pre, m.nd);
+        break;
+    case MK_PROOTS:
+        hipLaunchKernelGGL(k_multi_proots<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+                           (const MPRoots *)m.desc, pre, m.nd);
         break;
     default: set_error("multi_launch: unknown kind"); return TL_EINVAL;
     }

@@ -169,8 +169,21 @@ struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
 struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before)
     uint64_t A, stash, L, c, w;
 };
+struct MPRoots {   // one OP_PART_ROOTS of one array (stash: its first root)
+    uint64_t A, stash, base, stride, count;
+    uint32_t rp, classic, inv, pad;
+};
 
-enum MultiKind : uint32_t { MK_COPY, MK_LDS, MK_STREAM, MK_FUSED, MK_CHAIN, MK_BITREV, MK_WALK };
+enum MultiKind : uint32_t {
+    MK_COPY,
+    MK_LDS,
+    MK_STREAM,
+    MK_FUSED,
+    MK_CHAIN,
+    MK_BITREV,
+    MK_WALK,
+    MK_PROOTS
+};
 
 struct MultiLaunch {
     uint32_t kind, W, param;        // param: chained mode, or bitrev K
@@ -590,10 +603,21 @@ inline void plan_bst(Planner &P, uint64_t n) {
     }
     const uint32_t h = floor_log2_64(n);
     const uint32_t K = P.tu.bitrev_k ? P.tu.bitrev_k : 5u;
-    // n = 2^h: the one leaf (the smallest key) goes last and every other key
-    // one place left of the tree stored one place right.  As a gather that is
-    // a whole pass; folded into the last phase it is not.
-    if (P.tu.bst_fold && n == (1ull << h) && (n - 1) > P.cap() && h >= 2 * K + 2) {
+    // A non-perfect tree: 2^h - 1 internal keys and L >= 1 leaves.  The plan
+    // keeps the internal tree one place right, behind the smallest leaf:
+    // [leaf 0 | internal keys | leaves 1 .. L-1].  Key 1 is internal and the
+    // pairs (leaf, internal) from key 2 on are gathered (internals first),
+    // then the remaining internal keys rotate in front of the L - 1 leaves.
+    // For n = 2^h (L = 1) both are empty: the sorted keys are already there,
+    // and the leaf's move to the end, a whole pass as a gather, folds into
+    // the last phase.
+    const uint64_t nint_f = (1ull << h) - 1;
+    if (P.tu.bst_fold && n != (1ull << (h + 1)) - 1 && nint_f > P.cap() && h >= 2 * K + 2) {
+        const uint64_t L = n - nint_f;
+        if (L > 1) {
+            P.gather_dense({{2, 0, 1}}, L - 1, 1);
+            P.rotate_right({{L + 1, 0, 1}}, n - L - 1, nint_f - L);
+        }
         const uint32_t t0 = 2 * K;
         const uint64_t leaf = rowend_slots(h, K, t0);   // the leaf's stash slot
         Op one;
@@ -621,7 +645,7 @@ inline void plan_bst(Planner &P, uint64_t n) {
         two.p[2] = K;
         two.p[3] = BITREV_SHIFT;
         P.pl.ops.push_back(two);
-        P.copy(n - 1, 1, 1, false, leaf);
+        P.copy(nint_f, 1, 1, false, leaf);   // leaf 0 in front of the others
         return;
     }
     uint32_t d;
