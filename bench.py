@@ -451,6 +451,46 @@ def time_unpermute(args, A, layout, b, stream):
     return ev0.elapsed_time(ev1) / args.steps
 
 
+def other_layouts(args, A, keys, stream, rng, samples=20_000):
+    """BST, mvEB and classic vEB permutes (and inverses) of the headline's
+    n = 2^log2n uint64 keys in the same buffer: a sampled slot check against
+    the oracle's position functions, then ms per call over steps/2 calls"""
+    import torch
+    import tl_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    o = ol.oracle()
+    n = A.numel()
+    ranks = rng.integers(0, n, size=samples, dtype=np.int64)
+    want = keys[torch.from_numpy(ranks).to(A.device)].cpu().numpy()
+    steps = max(1, args.steps // 2)
+    out = {}
+    for layout in ("bst", "mveb", "veb"):
+        A.copy_(keys)
+        tl_amd.permute(layout, A, kind="u64")
+        torch.cuda.synchronize()
+        if layout == "bst":
+            pos = [o.or_btree_pos(int(x), n, 1) for x in ranks]
+        elif layout == "veb":
+            pos = [o.or_classic_pos(int(x), n) for x in ranks]
+        else:
+            pos = [o.or_mveb_pos(int(x), n) for x in ranks]
+        got = A[torch.tensor(pos, dtype=torch.int64, device=A.device)].cpu().numpy()
+        ok = bool(np.array_equal(got, want))
+        tl_amd.permute(layout, A, kind="u64", timed=False)
+        total, per = event_steps(lambda: tl_amd.permute(layout, A, kind="u64", timed=False),
+                                 steps, stream)
+        ms = total / steps
+        utotal, _ = event_steps(lambda: tl_amd.unpermute(layout, A, kind="u64", timed=False),
+                                steps, stream)
+        ums = utotal / steps
+        out[layout] = {"ms": round(ms, 4), "Gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                       "algorithmic_frac": round(2.0 * n * 8 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "unpermute_ms": round(ums, 4), "layout_sample": ok, "runs": steps}
+        log(f"[bench] {layout} permute {ms:.3f} ms, unpermute {ums:.3f} ms, sample check {ok}")
+    return out
+
+
 def run_layout(args, dev, layout):
     """Level-order BST or mvEB at n = 2^log2n uint64 (same key generator as the
     headline): in-place permute timing and plain query timing."""
@@ -551,6 +591,8 @@ def main():
                     help="learned model leaves (log2), the headline model")
     ap.add_argument("--log2leaves-big", type=int, default=20,
                     help="a second, larger learned model reported beside it (0: none)")
+    ap.add_argument("--skip-layouts", action="store_true",
+                    help="do not time the BST, mvEB and classic vEB permutes beside the B-tree")
     ap.add_argument("--skip-batch", action="store_true",
                     help="leave the batch permutations out of the headline line")
     ap.add_argument("--cpu-log2n", type=int, default=28, help="CPU baseline sample (log2 keys)")
@@ -627,6 +669,9 @@ def main():
                    "scratch_bytes": tl_amd.TL_NOALLOC_SCRATCH_BYTES,
                    "algorithmic_frac": round(2.0 * n * 8 / (na_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"[bench] allocation-free permute {na_ms:.3f} ms/step")
+
+    # ---- the other layouts' permutes on the same keys
+    layouts = None if args.skip_layouts else other_layouts(args, A, keys, stream, rng)
 
     # ---- rebuild the true layout for the queries
     A.copy_(keys)
@@ -740,6 +785,7 @@ def main():
         "permute_timings": perm_stats,
         "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
         "permute_fixed_scratch": noalloc,
+        "layouts": layouts,
         "queries": queries,
         "batch_permutes": batch,
         "checks": {"layout_sample": layout_ok, "plain_answers": plain_ok},
