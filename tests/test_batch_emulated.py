@@ -72,10 +72,13 @@ def test_planned_path_mixed_batch(planned):
     _run_planned(planned, 40, 19, 7)
 
 
-@pytest.mark.parametrize("one", ["2 u64 300007", "2 i32 1048576", "2 u64 1048575"])
-def test_planned_path_fused_slides(planned, one):
+@pytest.mark.parametrize("one", ["2 u64 300007", "2 i32 1048576", "2 u64 1048575",
+                                 "0 u64 300007", "0 i32 1048577", "0 u64 65536"])
+def test_planned_path_fused_forms(planned, one):
     """mvEB arrays whose top-level gather is a fused sparse slide (the leaf
     blocks' own gathers in the multi-array chained moves, their roots laid
-    out by the multi-array part-roots launch)"""
+    out by the multi-array part-roots launch), and BST arrays kept one place
+    right (the level heads' copies, the top read one place right, the
+    multi-array shifted level reversal)"""
     _run_planned(planned, 1, 19, 7, one)
 

@@ -860,8 +860,7 @@ static tl_status build_queues(Batch &B) {
 
 static tl_status build_planned(Batch &B, Ctx *c) {
     Tuning tu = tuning_for(c);
-    // the multi-array launches have no form of the BST kept one place right
-    tu.bst_fold = false;
+
     if (B.cfg.tile_k == 5 || B.cfg.tile_k == 6) tu.bitrev_k = B.cfg.tile_k;
     if (B.plan_lds) tu.lds_bytes = std::min<size_t>(B.plan_lds, c->lds_max - LDS_STATIC_RESERVE);
     std::vector<unsigned char> data;
@@ -1053,7 +1052,10 @@ static tl_status build_waves(Batch &B, Ctx *c) {
             G.desc.resize(at + dsize);
             memcpy(G.desc.data() + at, d, dsize);
             G.pre.push_back(G.pre.back() + tasks);
-            who[key].push_back(cur);
+            // one op may add several descriptors (the level heads' copies);
+            // its array advances once
+            std::vector<size_t> &w = who[key];
+            if (w.empty() || w.back() != cur) w.push_back(cur);
         };
         for (size_t li = 0; li < B.large.size(); li++) {
             const LargeItem &L = B.large[li];
@@ -1096,7 +1098,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 const uint64_t n = op.p[0], per = op.p[4];
                 const MLds d{A, runs, op.nitems, (uint32_t)op.nruns, (uint32_t)per, (uint32_t)n,
                              (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3],
-                             (uint32_t)((0xFFFFFFFFull + n) / n), (uint32_t)op.p[5]};
+                             (uint32_t)((0xFFFFFFFFull + n) / n), (uint32_t)op.p[5],
+                             (uint32_t)op.p[6], 0};
                 const uint32_t threads = (per * n >= 8192) ? 512 : 256;
                 add(ga, MK_LDS, W, 0, threads, &d, sizeof d, (op.nitems + per - 1) / per,
                     per * n * W + (per > 1 ? n * 4 : 0));
@@ -1124,16 +1127,28 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 add(ga, MK_CHAIN, W, mode, 256, &d, sizeof d, op.p[0], 0);
                 break;
             }
-            case OP_BITREV: {
-                const MBitrev d{A, (int64_t)op.p[0], (uint32_t)op.p[1], 0, 0, 0};
+            case OP_BITREV: {   // plain, or phase one of a kept tree (the same at base 0)
+                const MBitrev d{A, (int64_t)op.p[0], (uint32_t)op.p[1], 0, 0, 0, 0};
                 add(ga, MK_BITREV, W, (uint32_t)op.p[2], 256, &d, sizeof d,
                     1ull << (op.p[1] - 2 * op.p[2]), 0);
                 break;
             }
-            case OP_BITREV_LEVELS: {
-                const MBitrev d{A, 0, 0, (uint32_t)op.p[0], (uint32_t)op.p[1], 1};
-                add(ga, MK_BITREV, W, (uint32_t)op.p[2], 256, &d, sizeof d,
-                    (1ull << (op.p[1] - op.p[0])) - 1, 0);
+            case OP_BITREV_LEVELS: {   // plain, or a kept tree's levels shifted
+                const bool shift = op.p[3] == BITREV_SHIFT;
+                const MBitrev d{A, 0, 0, (uint32_t)op.p[0], (uint32_t)op.p[1], 1, stash};
+                add(ga, MK_BITREV, W, (uint32_t)op.p[2] | (shift ? 1u << 8 : 0u), 256, &d,
+                    sizeof d, (1ull << (op.p[1] - op.p[0])) - 1, 0);
+                break;
+            }
+            case OP_HEADS: {   // the level heads: one contiguous copy per level
+                const uint32_t t0 = (uint32_t)op.p[0], d = (uint32_t)op.p[1];
+                const uint32_t K = (uint32_t)op.p[2];
+                for (uint32_t t = t0; t < d; t++) {
+                    const uint64_t cnt = 1ull << (t - K);
+                    const MCopy c{stash + rowend_slots(t, K, t0) * W, A + (1ull << t) * W, 1, 1,
+                                  cnt};
+                    add(ga, MK_COPY, W, 0, 256, &c, sizeof c, (cnt + 2047) / 2048, 0);
+                }
                 break;
             }
             default: {

@@ -781,7 +781,7 @@ __global__ __launch_bounds__(512) void k_multi_lds(const MLds *__restrict__ d,
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MLds m = d[i];
     lds_body<T>((T *)m.A, (const uint64_t *)m.runs, m.nruns, m.nitems, m.per, m.n, m.kind, m.b,
-                m.rows, m.magic, m.inv, blockIdx.x - pre[i]);
+                m.rows, m.magic, m.inv, blockIdx.x - pre[i], m.shift);
 }
 
 // NT threads per partition (the stream's batches are NT * 8 keys; any batch
@@ -865,7 +865,9 @@ __global__ __launch_bounds__(256) void k_multi_proots(const MPRoots *__restrict_
         *slot = *st;
 }
 
-template <typename T, int K>
+// SHIFT = 1: every descriptor is the levels of a BST kept one place right,
+// written one slot left (BITREV_SHIFT)
+template <typename T, int K, int SHIFT = 0>
 __global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict__ d,
                                                       const uint64_t *__restrict__ pre,
                                                       uint32_t nd) {
@@ -874,7 +876,9 @@ __global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict_
     const uint32_t i = find_desc(pre, nd, t);
     const MBitrev m = d[i];
     const uint64_t g = t - pre[i];
-    if (m.levels) {
+    if constexpr (SHIFT == 1) {
+        level_tile<T, K, 1>((T *)m.A, m.t0, m.d, g, (T *)m.stash);
+    } else if (m.levels) {
         const uint32_t j = 63u - (uint32_t)__clzll(g + 1);
         const uint32_t lev = m.t0 + j;
         bitrev_tile<T, K>((T *)m.A, (int64_t)((1ull << lev) - 1), lev, g - ((1ull << j) - 1));
@@ -947,12 +951,23 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     case MK_BITREV: {
         const uint32_t gx = (uint32_t)std::min<uint64_t>(m.tasks, 1u << 20);
         const uint32_t gy = (uint32_t)((m.tasks + gx - 1) / gx);
-        if (m.param == 6)
-            hipLaunchKernelGGL((k_multi_bitrev<T, 6>), dim3(gx, gy), dim3(256), 0, s,
-                               (const MBitrev *)m.desc, pre, m.nd);
-        else
-            hipLaunchKernelGGL((k_multi_bitrev<T, 5>), dim3(gx, gy), dim3(256), 0, s,
-                               (const MBitrev *)m.desc, pre, m.nd);
+        // param: K, and 1 << 8 for the shifted levels of kept trees
+        const bool shift = (m.param >> 8) == 1;
+        if ((m.param & 0xff) == 6) {
+            if (shift)
+                hipLaunchKernelGGL((k_multi_bitrev<T, 6, 1>), dim3(gx, gy), dim3(256), 0, s,
+                                   (const MBitrev *)m.desc, pre, m.nd);
+            else
+                hipLaunchKernelGGL((k_multi_bitrev<T, 6>), dim3(gx, gy), dim3(256), 0, s,
+                                   (const MBitrev *)m.desc, pre, m.nd);
+        } else {
+            if (shift)
+                hipLaunchKernelGGL((k_multi_bitrev<T, 5, 1>), dim3(gx, gy), dim3(256), 0, s,
+                                   (const MBitrev *)m.desc, pre, m.nd);
+            else
+                hipLaunchKernelGGL((k_multi_bitrev<T, 5>), dim3(gx, gy), dim3(256), 0, s,
+                                   (const MBitrev *)m.desc, pre, m.nd);
+        }
         break;
     }
     case MK_WALK:

@@ -151,7 +151,7 @@ struct MCopy {   // dst[i*dstride] = src[i*sstride], i < count
 };
 struct MLds {    // one OP_LDS of one array
     uint64_t A, runs, nitems;
-    uint32_t nruns, per, n, kind, b, rows, magic, inv;
+    uint32_t nruns, per, n, kind, b, rows, magic, inv, shift, pad;
 };
 struct MStream {   // one OP_SPARSE_STREAM of one array
     uint64_t A, runs;
@@ -165,6 +165,7 @@ struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
     uint64_t A;
     int64_t base;
     uint32_t D, t0, d, levels;
+    uint64_t stash;   // BITREV_SHIFT levels: the array's level heads
 };
 struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before)
     uint64_t A, stash, L, c, w;
