@@ -451,6 +451,15 @@ def time_unpermute(args, A, layout, b, stream):
     return ev0.elapsed_time(ev1) / args.steps
 
 
+def layout_traffic(layout, n):
+    """HBM bytes per permute call of the BST or mvEB at n = 2^30 from the
+    committed PMC summaries (None for other sizes and layouts)"""
+    if n != 1 << 30 or layout not in ("bst", "mveb"):
+        return None
+    t = measured_traffic(f"{layout}_n2p30_u64_permute_traffic.txt")
+    return None if t is None else round(t)
+
+
 def other_layouts(args, A, keys, stream, rng, samples=20_000):
     """BST, mvEB and classic vEB permutes (and inverses) of the headline's
     n = 2^log2n uint64 keys in the same buffer: a sampled slot check against
@@ -484,8 +493,12 @@ def other_layouts(args, A, keys, stream, rng, samples=20_000):
         utotal, _ = event_steps(lambda: tl_amd.unpermute(layout, A, kind="u64", timed=False),
                                 steps, stream)
         ums = utotal / steps
+        traffic = layout_traffic(layout, n)
         out[layout] = {"ms": round(ms, 4), "Gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
                        "algorithmic_frac": round(2.0 * n * 8 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "traffic": traffic,
+                       "traffic_frac": None if traffic is None
+                       else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "unpermute_ms": round(ums, 4), "layout_sample": ok, "runs": steps}
         log(f"[bench] {layout} permute {ms:.3f} ms, unpermute {ums:.3f} ms, sample check {ok}")
     return out
@@ -570,7 +583,8 @@ def run_layout(args, dev, layout):
         "unpermute": {"ms": round(unperm_ms, 4), "Gkeys_s": round(n / (unperm_ms * 1e-3) / 1e9, 3)},
         "checks": {"layout_sample": layout_ok, "answers": ok},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": layout_traffic(layout, n),
                      "kernel": f"whole tl_permute_{layout} call (all its launches)"},
         "cpu_baseline": None,
     }
