@@ -105,6 +105,44 @@ tl_status tl_permute_veb(void *dev_A, uint64_t n, tl_dtype dtype, void *stream, 
 tl_status tl_permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
                                    void *stream);
 
+/* ---- the involution family ---------------------------------------------- */
+
+/* The reference's second set of permutation programs, in which every pass
+ * swaps each key with a closed-form partner (an involution).  Same layouts as
+ * tl_permute_{bst,btree,mveb}; more passes and scattered partners, so slower
+ * (the gather / cycle-leader path above is the fast one).  No scratch beyond
+ * the LDS tables of the mvEB's in-LDS base cases.  Replace
+ *   timePermuteBST<T>(dev_A, n)       (gpu/src/bst-involutions.cuh:167),
+ *   timePermuteBtree<T>(dev_A, n, b)  (gpu/src/btree-involutions.cuh:151),
+ *   timePermutevEB<T>(dev_A, n)       (gpu/src/mveb-involutions.cuh:503).
+ * The BST's perfect part (phaseOne, phaseTwo) runs as tl_permute_bst's tiled
+ * bit reversals. */
+tl_status tl_permute_bst_involutions(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                     float *ms);
+tl_status tl_permute_btree_involutions(void *dev_A, uint64_t n, uint64_t b, tl_dtype dtype,
+                                       void *stream, float *ms);
+tl_status tl_permute_mveb_involutions(void *dev_A, uint64_t n, tl_dtype dtype, void *stream,
+                                      float *ms);
+
+/* Their primitives, each two involution passes over dev_A[0..n):
+ * k-way shuffle of n = d*k keys, key i*d + j -> j*k + i
+ *   (shuffle_dk_phaseOne/Two, gpu/src/involutions.cuh:84,106);
+ * its inverse, every k-th key gathered: key j*k + i -> i*d + j
+ *   (unshuffle_dk, cpu/src/involutions.h:161);
+ * k-way unshuffle of n = k^d - 1 keys: a stable partition by (i+1) mod k,
+ *   residue 0 first, then 1, ..., k-1 (unshuffle_phaseOne/Two,
+ *   gpu/src/involutions.cuh:38,61);
+ * rotation right by k <= n by two reversals (shift_right_phaseOne/Two,
+ *   gpu/src/common.cuh:62-110). */
+tl_status tl_shuffle_dk(void *dev_A, uint64_t n, uint64_t k, tl_dtype dtype, void *stream,
+                        float *ms);
+tl_status tl_unshuffle_dk(void *dev_A, uint64_t n, uint64_t k, tl_dtype dtype, void *stream,
+                          float *ms);
+tl_status tl_unshuffle(void *dev_A, uint64_t n, uint64_t k, tl_dtype dtype, void *stream,
+                       float *ms);
+tl_status tl_shift_right(void *dev_A, uint64_t n, uint64_t k, tl_dtype dtype, void *stream,
+                         float *ms);
+
 /* ---- inverse permutations: layout -> sorted order, in place ------------- */
 
 /* The exact inverses of tl_permute_{bst,btree,mveb} for the same (n, b): the

@@ -139,6 +139,41 @@ double timePermutevEB(T *dev_A, uint64_t n) {
     return ms;
 }
 
+// The involution programs define the same three names
+// (gpu/src/bst-involutions.cuh:167, gpu/src/btree-involutions.cuh:151,
+// gpu/src/mveb-involutions.cuh:503); a driver built on them calls these.
+namespace involutions {
+template <typename T>
+float timePermuteBST(T *dev_A, uint64_t n) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_bst_involutions(dev_A, n, tl_detail::dtype_of<T>::v, nullptr,
+                                                  &ms),
+                       "involutions::timePermuteBST"))
+        return -1.f;
+    return ms;
+}
+
+template <typename T>
+double timePermuteBtree(T *dev_A, uint64_t n, uint64_t b) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_btree_involutions(dev_A, n, b, tl_detail::dtype_of<T>::v,
+                                                    nullptr, &ms),
+                       "involutions::timePermuteBtree"))
+        return -1.0;
+    return ms;
+}
+
+template <typename T>
+double timePermutevEB(T *dev_A, uint64_t n) {
+    float ms = 0.f;
+    if (!tl_detail::ok(tl_permute_mveb_involutions(dev_A, n, tl_detail::dtype_of<T>::v, nullptr,
+                                                   &ms),
+                       "involutions::timePermutevEB"))
+        return -1.0;
+    return ms;
+}
+}   // namespace involutions
+
 // ---- queries (hits drawn from the host layout A; kernel time in ms) ---------
 
 template <typename T>

@@ -9,7 +9,9 @@
 //     children are nodes k*(b+1)+1 .. k*(b+1)+b+1);
 //   * the mvEB layout by a digest of its rank sequence, printed as
 //     "digest <type> <n> <hex>" for the test to compare with the oracle;
-//   * every query answer (all hits) equal to the slot that holds the key.
+//   * every query answer (all hits) equal to the slot that holds the key;
+//   * the involution programs' permutations (namespace involutions) by the
+//     same checks ("digest-inv" lines for the mvEB).
 // Exit status 0 when every check passes.
 #include <stdio.h>
 #include <stdlib.h>
@@ -138,6 +140,34 @@ static int run(uint64_t n, const char *tname) {
         tl_detail::answer_sink() = nullptr;
         printf("layout %d %s query: ", layout, tname);
         printQueryTimings(n, nq, qtime);
+    }
+    // the involution programs' names (tree_layouts.hpp, namespace involutions):
+    // the same layouts, the mvEB's digest printed as "digest-inv"
+    for (int layout = 0; layout < 3; layout++) {
+        initSortedList<T>(A.data(), n);
+        if (hipMemcpy(dev_A, A.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return 1;
+        const double t = layout == 0 ? involutions::timePermuteBST<T>(dev_A, n)
+                       : layout == 1 ? involutions::timePermuteBtree<T>(dev_A, n, 16)
+                                     : involutions::timePermutevEB<T>(dev_A, n);
+        if (t < 0) bad++;
+        if (hipMemcpy(A.data(), dev_A, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+            return 1;
+        std::vector<uint64_t> rank(n);
+        for (uint64_t s = 0; s < n; s++) rank[s] = (uint64_t)A[s] - 1;
+        if (layout < 2) {
+            std::vector<uint64_t> expect(n, n);
+            uint64_t next = 0;
+            inorder(0, layout == 0 ? 1 : 16, n, next, expect);
+            uint64_t wrong = 0;
+            for (uint64_t s = 0; s < n; s++) wrong += expect[s] != rank[s];
+            if (wrong) bad++;
+            printf("involutions layout %d %s n=%llu: %.3f ms, %llu slots differ\n", layout, tname,
+                   (unsigned long long)n, t, (unsigned long long)wrong);
+        } else {
+            printf("digest-inv %s %llu %016llx\n", tname, (unsigned long long)n,
+                   (unsigned long long)digest(rank));
+        }
     }
     // the comparator on the sorted list (gpu/test/query-binarysearch-fixedN.cu):
     // the answer of a hit is its rank

@@ -10,7 +10,11 @@ chained moves) of perfect and non-perfect trees, u32 and u64; the classic
 vEB's fused slides differ only in the slot function, and their plans are
 replayed in test_plan.py (its host oracle is too slow for this file).  The
 BST cases cover the tree kept one place right (2^16 keys, and 300,007 keys:
-gather, rotation, then the same phases)."""
+gather, rotation, then the same phases).  The involution family
+(tl_involution.hip) runs the same way: its three permutations against the
+oracle's layouts (B-trees of b = 1, 2, 3, 8, 16; the mvEB's global rounds and
+in-LDS base cases) and its shuffle, unshuffle and rotation primitives against
+their closed forms."""
 import os
 import subprocess
 
@@ -31,10 +35,23 @@ def emulator():
     ("mveb", 4, [300007]),
     ("bst", 8, [65536, 300007]),
     ("btree", 4, [300007]),
+    ("bst_inv", 8, [4097, 300007]),
+    ("btree_inv", 4, [4914, 30000]),
+    ("mveb_inv", 4, [70000]),
+    ("mveb_inv", 8, [4095, 20000]),
+    ("prims", 8, [5040]),
+    ("prims", 4, [4096]),
 ])
 def test_permute_emulated(emulator, layout, width, sizes):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
     r = subprocess.run([emulator, layout, str(width)] + [str(n) for n in sizes],
                        capture_output=True, text=True, timeout=900, env=env)
     assert r.returncode == 0, r.stderr[-4000:]
-    assert r.stderr.count("0 wrong, inverse 0 wrong") == len(sizes), r.stderr[-4000:]
+    if layout == "prims":
+        assert r.stderr.count(": 0 wrong") == len(sizes), r.stderr[-4000:]
+    elif layout.endswith("_inv"):
+        # the B-tree cases run b = 1, 2, 3, 8, 16 per size
+        per = 5 if layout == "btree_inv" else 1
+        assert r.stderr.count(", 0 wrong") == per * len(sizes), r.stderr[-4000:]
+    else:
+        assert r.stderr.count("0 wrong, inverse 0 wrong") == len(sizes), r.stderr[-4000:]

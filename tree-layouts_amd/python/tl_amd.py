@@ -64,6 +64,21 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_float)],
     "tl_permute_veb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                        ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_bst_involutions": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_btree_involutions": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_float)],
+    "tl_permute_mveb_involutions": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_shuffle_dk": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_unshuffle_dk": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_unshuffle": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
+    "tl_shift_right": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)],
     "tl_unpermute_veb": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
                          ctypes.POINTER(ctypes.c_float)],
     "tl_query_bst": [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
@@ -305,6 +320,51 @@ def permute(layout, A, b=16, kind=None, stream=None, timed=True):
     if layout == "veb":
         return permute_veb(A, kind, stream, timed)
     return permute_mveb(A, kind, stream, timed)
+
+
+def permute_involutions(layout, A, b=16, kind=None, stream=None, timed=True):
+    """The involution family (tl_permute_{bst,btree,mveb}_involutions): the
+    same layouts as permute(), by swap passes with closed-form partners."""
+    k = _kind(A, kind)
+    L = lib()
+    if layout == "bst":
+        return _timed_call(L.tl_permute_bst_involutions, _ptr(A), A.numel(), DTYPE[k],
+                           _stream(stream), timed=timed)
+    if layout == "btree":
+        return _timed_call(L.tl_permute_btree_involutions, _ptr(A), A.numel(), b, DTYPE[k],
+                           _stream(stream), timed=timed)
+    if layout == "mveb":
+        return _timed_call(L.tl_permute_mveb_involutions, _ptr(A), A.numel(), DTYPE[k],
+                           _stream(stream), timed=timed)
+    raise ValueError(f"no involution permutation for layout {layout!r}")
+
+
+def shuffle_dk(A, k, kind=None, stream=None, timed=True):
+    """k-way shuffle of A (numel a multiple of k): key i*d + j -> slot j*k + i."""
+    t = _kind(A, kind)
+    return _timed_call(lib().tl_shuffle_dk, _ptr(A), A.numel(), k, DTYPE[t], _stream(stream),
+                       timed=timed)
+
+
+def unshuffle_dk(A, k, kind=None, stream=None, timed=True):
+    """Inverse of shuffle_dk: every k-th key gathered."""
+    t = _kind(A, kind)
+    return _timed_call(lib().tl_unshuffle_dk, _ptr(A), A.numel(), k, DTYPE[t], _stream(stream),
+                       timed=timed)
+
+
+def unshuffle(A, k, kind=None, stream=None, timed=True):
+    """k-way unshuffle of numel = k^d - 1 keys: stable partition by (i+1) mod k."""
+    t = _kind(A, kind)
+    return _timed_call(lib().tl_unshuffle, _ptr(A), A.numel(), k, DTYPE[t], _stream(stream),
+                       timed=timed)
+
+
+def shift_right(A, k, kind=None, stream=None, timed=True):
+    """Rotation of A right by k by two reversal passes."""
+    t = _kind(A, kind)
+    return _timed_call(lib().tl_shift_right, _ptr(A), A.numel(), k, DTYPE[t], _stream(stream),
+                       timed=timed)
 
 
 def unpermute(layout, A, b=16, kind=None, stream=None, timed=True):
