@@ -501,6 +501,22 @@ def other_layouts(args, A, keys, stream, rng, samples=20_000):
                        else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "unpermute_ms": round(ums, 4), "layout_sample": ok, "runs": steps}
         log(f"[bench] {layout} permute {ms:.3f} ms, unpermute {ums:.3f} ms, sample check {ok}")
+    # the involution family: same layouts by swap passes, one timed call each
+    # from the sorted keys, with the same sampled slot check
+    inv = {}
+    for layout, b in (("bst", 1), ("btree", 16), ("mveb", 0)):
+        A.copy_(keys)
+        ms = tl_amd.permute_involutions(layout, A, b=b, kind="u64")
+        if layout == "mveb":
+            pos = [o.or_mveb_pos(int(x), n) for x in ranks]
+        else:
+            pos = [o.or_btree_pos(int(x), n, b) for x in ranks]
+        got = A[torch.tensor(pos, dtype=torch.int64, device=A.device)].cpu().numpy()
+        ok = bool(np.array_equal(got, want))
+        inv[layout] = {"ms": round(ms, 3), "Gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                       "layout_sample": ok}
+        log(f"[bench] {layout} involutions permute {ms:.3f} ms, sample check {ok}")
+    out["involutions"] = inv
     return out
 
 

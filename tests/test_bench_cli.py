@@ -21,7 +21,13 @@ def test_traffic_from_committed_profiles():
     bt = bench.measured_traffic("btree_b16_n2p30_u64_permute_traffic.txt")
     bp = bench.measured_traffic("batch_permutes_traffic.txt")
     # B-tree b=16 at 2^30 moves about 2.2 times its 17.18 GB of algorithmic
-    # bytes; the batch's permutations about 2.6 times their 21.3 GB
+    # bytes, the batch's permutations about 2.2 times their 21.3 GB, the BST
+    # about 2.05 passes and the mvEB about one
     assert bt is not None and 2.0 * 17.18e9 < bt < 2.5 * 17.18e9
-    assert bp is not None and 2.3 * 21.3e9 < bp < 3.0 * 21.3e9
+    assert bp is not None and 1.8 * 21.3e9 < bp < 3.0 * 21.3e9
+    bst = bench.layout_traffic("bst", 1 << 30)
+    mveb = bench.layout_traffic("mveb", 1 << 30)
+    assert bst is not None and 1.8 * 17.18e9 < bst < 2.4 * 17.18e9
+    assert mveb is not None and 0.95 * 17.18e9 < mveb < 1.2 * 17.18e9
+    assert bench.layout_traffic("mveb", 1 << 20) is None
     assert bench.measured_traffic("no_such_profile.txt") is None

@@ -887,12 +887,12 @@ __global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict_
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_multi_walk(const MWalk *__restrict__ d,
-                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
+template <typename T, int BS = 256>
+__global__ __launch_bounds__(BS) void k_multi_walk(const MWalk *__restrict__ d,
+                                                   const uint64_t *__restrict__ pre, uint32_t nd) {
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MWalk m = d[i];
-    const uint64_t j = (blockIdx.x - pre[i]) * 256 + threadIdx.x;
+    const uint64_t j = (blockIdx.x - pre[i]) * BS + threadIdx.x;
     if (j < m.w) walk_lane<T, 4>((T *)m.A, m.L, m.c, m.w, (const T *)m.stash, j);
 }
 
@@ -971,8 +971,12 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
         break;
     }
     case MK_WALK:
-        hipLaunchKernelGGL(k_multi_walk<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
-                           (const MWalk *)m.desc, pre, m.nd);
+        if (m.threads == 1024)
+            hipLaunchKernelGGL((k_multi_walk<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024), 0, s,
+                               (const MWalk *)m.desc, pre, m.nd);
+        else
+            hipLaunchKernelGGL(k_multi_walk<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+                               (const MWalk *)m.desc, pre, m.nd);
         break;
     case MK_PROOTS:
         hipLaunchKernelGGL(k_multi_proots<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
