@@ -284,3 +284,67 @@ uint64_t or_splitmix64(uint64_t seed, uint64_t i) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+
+/* ---- involution helpers (cpu/src/involutions.cpp) ----------------------- */
+
+/* cpu/src/involutions.cpp:44-63: digits taken from the top (power k down to
+ * 1) and laid down from the bottom */
+uint64_t or_rev_base_d(uint64_t x, uint64_t base, uint64_t d) {
+    if (d == 0) return x;
+    uint64_t r = 0, j = 1, k = or_ipow(base, (uint32_t)(d - 1));
+    for (uint64_t i = d - 1; i > 0; --i) {
+        const uint64_t digit = x / k;
+        r += digit * j;
+        j *= base;
+        x -= digit * k;
+        k /= base;
+    }
+    return r + x * j;
+}
+
+/* cpu/src/involutions.cpp:66-80: the top d - b digits kept in place */
+uint64_t or_rev_base_b(uint64_t x, uint64_t base, uint64_t d, uint64_t b) {
+    uint64_t r = 0, k = or_ipow(base, (uint32_t)(d - 1));
+    for (uint64_t i = d - 1; i + 1 > b; --i) {   /* i >= b without wrapping at b = 0 */
+        const uint64_t digit = x / k;
+        r += digit * k;
+        x -= digit * k;
+        k /= base;
+        if (i == 0) break;
+    }
+    return r + or_rev_base_d(x, base, b);
+}
+
+/* cpu/src/involutions.cpp:83-93 */
+static uint64_t or_gcd(uint64_t a, uint64_t b) {
+    while (a) {
+        const uint64_t c = a;
+        a = b % a;
+        b = c;
+    }
+    return b;
+}
+
+/* cpu/src/involutions.cpp:96-109, recursive: returns g, with *x a coefficient
+ * of a (a * x + b * y = g) */
+static uint64_t or_egcd(uint64_t a, uint64_t b, int64_t *x, int64_t *y) {
+    if (a == 0) {
+        *x = 0;
+        *y = 1;
+        return b;
+    }
+    int64_t x1, y1;
+    const uint64_t g = or_egcd(b % a, a, &x1, &y1);
+    *x = y1 - (int64_t)(b / a) * x1;
+    *y = x1;
+    return g;
+}
+
+/* cpu/src/involutions.cpp:112-121 */
+uint64_t or_involution(uint64_t r, uint64_t x, uint64_t m) {
+    const uint64_t g = or_gcd(x, m);
+    int64_t inv, y;
+    or_egcd(x / g, m / g, &inv, &y);
+    if (inv < 0) inv += (int64_t)(m / g);
+    return g * (r * (uint64_t)inv % (m / g));
+}

@@ -7,8 +7,9 @@
  *
  * Two independent families live here:
  *   (1) a restatement, in plain C + OpenMP, of the reference's CPU algorithms
- *       (cycle-leader gathers, reversal shifts, BST involutions, queries) and
- *       of the reference's GPU mvEB recursion, which has no CPU twin;
+ *       (cycle-leader gathers, reversal shifts, BST involutions, queries, the
+ *       involution programs and their shuffles) and of the reference's GPU
+ *       mvEB recursions (cycles and involutions), which have no CPU twin;
  *   (2) constructions that do not share code with (1): the in-order DFS fill
  *       of the heap-indexed (b+1)-ary node tree (BST = b 1), the rank->slot
  *       closed forms, and the structural mvEB recursion.
@@ -75,6 +76,27 @@ int or_permute_mveb_u32(uint32_t *A, uint64_t n);
 int or_permute_mveb_u64(uint64_t *A, uint64_t n);
 int or_permute_classic_u32(uint32_t *A, uint64_t n, int p);
 int or_permute_classic_u64(uint64_t *A, uint64_t n, int p);
+
+/* ---- the involution family (family 1): cpu/src/involutions.{h,cpp},
+ * cpu/src/bst-involutions.h, cpu/src/btree-involutions.h and, for the mvEB,
+ * the host recursion of gpu/src/mveb-involutions.cuh (no CPU twin) ------- */
+/* the low d base-`base` digits of x reversed (cpu/src/involutions.cpp:44-63),
+ * the low b of d digits reversed (:66-80) */
+uint64_t or_rev_base_d(uint64_t x, uint64_t base, uint64_t d);
+uint64_t or_rev_base_b(uint64_t x, uint64_t base, uint64_t d, uint64_t b);
+/* g * (r * inverse(x/g) mod (m/g)), g = gcd(x, m) (cpu/src/involutions.cpp:83-121) */
+uint64_t or_involution(uint64_t r, uint64_t x, uint64_t m);
+#define OR_INVDECL(SUF, T)                                                                   \
+    void or_unshuffle_##SUF(T *A, uint64_t k, uint64_t n, uint64_t d);                      \
+    void or_shuffle_dk_##SUF(T *A, uint64_t k, uint64_t n);                                 \
+    void or_unshuffle_dk_##SUF(T *A, uint64_t k, uint64_t n);                               \
+    void or_rotate_right_##SUF(T *A, uint64_t n, uint64_t k);                               \
+    int or_permute_bst_inv_##SUF(T *A, uint64_t n);                                         \
+    int or_permute_btree_inv_##SUF(T *A, uint64_t n, uint64_t b);                           \
+    int or_permute_mveb_inv_##SUF(T *A, uint64_t n);
+OR_INVDECL(u32, uint32_t)
+OR_INVDECL(u64, uint64_t)
+#undef OR_INVDECL
 
 /* ---- restated queries: answers[i] = slot of queries[i] or n ------------ */
 #define OR_QDECL(SUF, T)                                                                    \

@@ -55,6 +55,17 @@ def oracle():
             getattr(o, f"or_permute_btree_{w}").argtypes = [VP, C, C, I]
             getattr(o, f"or_permute_mveb_{w}").argtypes = [VP, C]
             getattr(o, f"or_permute_classic_{w}").argtypes = [VP, C, I]
+        o.or_rev_base_d.argtypes = [C, C, C]; o.or_rev_base_d.restype = C
+        o.or_rev_base_b.argtypes = [C, C, C, C]; o.or_rev_base_b.restype = C
+        o.or_involution.argtypes = [C, C, C]; o.or_involution.restype = C
+        for w in ("u32", "u64"):
+            getattr(o, f"or_unshuffle_{w}").argtypes = [VP, C, C, C]
+            getattr(o, f"or_shuffle_dk_{w}").argtypes = [VP, C, C]
+            getattr(o, f"or_unshuffle_dk_{w}").argtypes = [VP, C, C]
+            getattr(o, f"or_rotate_right_{w}").argtypes = [VP, C, C]
+            getattr(o, f"or_permute_bst_inv_{w}").argtypes = [VP, C]
+            getattr(o, f"or_permute_btree_inv_{w}").argtypes = [VP, C, C]
+            getattr(o, f"or_permute_mveb_inv_{w}").argtypes = [VP, C]
         for k in ("u32", "u64", "i32", "i64"):
             getattr(o, f"or_search_bst_{k}").argtypes = [VP, C, VP, C, VP, I]
             getattr(o, f"or_search_btree_{k}").argtypes = [VP, C, C, VP, C, VP, I]
@@ -79,6 +90,11 @@ def ref():
         r.ref_search_btree_u64.argtypes = [VP, C, C, VP, C, VP, I]
         r.ref_search_binary_u64.argtypes = [VP, C, VP, C, VP, I]
         r.ref_build_table_classic.argtypes = [VP, C, ctypes.c_uint32]
+        r.ref_permute_bst_inv_u64.argtypes = [VP, C, I]
+        r.ref_permute_btree_inv_u64.argtypes = [VP, C, C, I]
+        r.ref_unshuffle_u64.argtypes = [VP, C, C, C]
+        r.ref_shuffle_dk_u64.argtypes = [VP, C, C]
+        r.ref_unshuffle_dk_u64.argtypes = [VP, C, C]
         _r = r
     return _r
 
@@ -108,6 +124,20 @@ def permute(layout, a, b=16, p=1):
     else:
         rc = getattr(o, f"or_permute_mveb_{w}")(_p(v), v.size)
     assert rc == 0
+    return a
+
+
+def permute_involutions(layout, a, b=16):
+    """The oracle's restatement of the involution programs, in place."""
+    o = oracle()
+    v = a.view(np.uint32 if a.dtype.itemsize == 4 else np.uint64)
+    w = width_suffix(v)
+    if layout == "bst":
+        getattr(o, f"or_permute_bst_inv_{w}")(_p(v), v.size)
+    elif layout == "btree":
+        getattr(o, f"or_permute_btree_inv_{w}")(_p(v), v.size, b)
+    else:
+        getattr(o, f"or_permute_mveb_inv_{w}")(_p(v), v.size)
     return a
 
 

@@ -288,3 +288,102 @@ def test_live_reference_agrees():
             a = np.arange(n, dtype=np.uint64)
             ol.ref().ref_permute_btree_u64(a.ctypes.data_as(ol.VP), n, b, 8)
             assert np.array_equal(a, ol.layout_ranks("btree", n, b)), (b, n)
+
+
+# ---- the involution family ----------------------------------------------
+
+def _prim(name, a, *args):
+    o = ol.oracle()
+    getattr(o, f"or_{name}_{ol.width_suffix(a)}")(a.ctypes.data_as(ol.VP), *args)
+    return a
+
+
+@pytest.mark.parametrize("dt", [np.uint32, np.uint64])
+def test_oracle_involution_primitives_closed_forms(dt):
+    """The restated shuffle primitives against what the mathematics fixes:
+    the k-way shuffle of n = d*k keys is the transpose of a k x d matrix, its
+    inverse the transpose of a d x k one, the unshuffle of k^d - 1 keys the
+    stable partition by (i+1) mod k, the rotation numpy's roll."""
+    for k, d in [(1, 9), (2, 1), (2, 7), (3, 5), (4, 4), (17, 6), (16, 15), (5, 100)]:
+        n = k * d
+        a = np.arange(n)
+        got = _prim("shuffle_dk", np.arange(n, dtype=dt), k, n)
+        assert np.array_equal(got, a.reshape(k, d).T.ravel()), (k, d)
+        got = _prim("unshuffle_dk", np.arange(n, dtype=dt), k, n)
+        assert np.array_equal(got, a.reshape(d, k).T.ravel()), (k, d)
+    for k, d in [(2, 1), (2, 9), (3, 5), (17, 3), (16, 2), (4, 2), (256, 2)]:
+        n = k ** d - 1
+        i = np.arange(n)
+        exp = np.concatenate([i[(i + 1) % k == r] for r in range(k)])
+        assert np.array_equal(_prim("unshuffle", np.arange(n, dtype=dt), k, n, d), exp), (k, d)
+    for n in (1, 2, 10, 1001):
+        for k in sorted({0, 1, n // 3, n - 1, n}):
+            got = _prim("rotate_right", np.arange(n, dtype=dt), n, k)
+            assert np.array_equal(got, np.roll(np.arange(n), k)), (n, k)
+    # digit reversal and the modular involution on their own
+    o = ol.oracle()
+    assert o.or_rev_base_d(int("1203", 4), 4, 4) == int("3021", 4)
+    assert o.or_rev_base_b(int("1203", 4), 4, 4, 2) == int("1230", 4)
+    for m in (12, 17, 100, 255):
+        for x in range(1, m):
+            y = o.or_involution(1, x, m)
+            assert o.or_involution(1, y, m) == x                 # an involution
+            g = np.gcd(x, m)
+            assert (x * y) % m == (g * g) % m, (x, m)            # y = g^2 / x (mod m)
+
+
+def test_oracle_involution_permutes_match_layouts():
+    """The restated involution programs lay out exactly the trees of the
+    independent in-order and structural constructions (family 2), including
+    the B-tree sizes where the reference's empty shuffle does not end."""
+    sizes = list(range(0, 400)) + [1000, 4095, 4096, 4913, 4914, 4929, 20000, 65535, 65536,
+                                   70000, 83521, 100003]
+    for n in sizes:
+        a = ol.permute_involutions("bst", np.arange(n, dtype=np.uint64))
+        assert np.array_equal(a, ol.layout_ranks("bst", n, 1)), ("bst", n)
+        a = ol.permute_involutions("mveb", np.arange(n, dtype=np.uint64))
+        assert np.array_equal(a, ol.layout_ranks("mveb", n)), ("mveb", n)
+        for b in (1, 2, 3, 8, 16):
+            if n > 20000 and b != 16:
+                continue
+            a = ol.permute_involutions("btree", np.arange(n, dtype=np.uint32), b)
+            assert np.array_equal(a, ol.layout_ranks("btree", n, b).astype(np.uint32)), (b, n)
+
+
+def _btree_inv_hangs(n, b):
+    """n in [(b+1)^h, (b+1)^h + b - 2]: the reference's B-tree involution
+    program shuffles an empty range there, and its loop over [1, n - 1) with
+    n = 0 does not end (cpu/src/involutions.h:162-164)"""
+    p = b + 1
+    while p * (b + 1) <= n:
+        p *= b + 1
+    return p <= n <= p + b - 2
+
+
+@pytest.mark.skipif(not os.path.exists(ol.REF_SO), reason="reference not compiled in this environment")
+def test_live_reference_involutions_agree():
+    r = ol.ref()
+    for n in (1, 2, 3, 6, 7, 8, 100, 1000, 4095, 4096, 65535, 100003, 300007):
+        a = np.arange(n, dtype=np.uint64)
+        r.ref_permute_bst_inv_u64(a.ctypes.data_as(ol.VP), n, 1)
+        assert np.array_equal(a, ol.permute_involutions("bst", np.arange(n, dtype=np.uint64))), n
+    for b in (1, 2, 8, 16):
+        for n in (b + 1, 100, 500, 5000, 17 ** 4 - 1, 17 ** 4 + 20, 100003):
+            if _btree_inv_hangs(n, b):
+                continue
+            a = np.arange(n, dtype=np.uint64)
+            r.ref_permute_btree_inv_u64(a.ctypes.data_as(ol.VP), n, b, 1)
+            exp = ol.permute_involutions("btree", np.arange(n, dtype=np.uint64), b)
+            assert np.array_equal(a, exp), (b, n)
+    for k, d in [(2, 8), (3, 5), (17, 3)]:
+        n = k ** d - 1
+        a = np.arange(n, dtype=np.uint64)
+        r.ref_unshuffle_u64(a.ctypes.data_as(ol.VP), k, n, d)
+        assert np.array_equal(a, _prim("unshuffle", np.arange(n, dtype=np.uint64), k, n, d))
+    for k, n in [(2, 10), (3, 300), (17, 17 * 99), (16, 4096)]:
+        a = np.arange(n, dtype=np.uint64)
+        r.ref_shuffle_dk_u64(a.ctypes.data_as(ol.VP), k, n)
+        assert np.array_equal(a, _prim("shuffle_dk", np.arange(n, dtype=np.uint64), k, n))
+        a = np.arange(n, dtype=np.uint64)
+        r.ref_unshuffle_dk_u64(a.ctypes.data_as(ol.VP), k, n)
+        assert np.array_equal(a, _prim("unshuffle_dk", np.arange(n, dtype=np.uint64), k, n))
