@@ -607,6 +607,37 @@ def run_layout(args, dev, layout):
     print(json.dumps(out), flush=True)
 
 
+def run_involutions(args, dev):
+    """The involution family at n = 2^log2n uint64: each permutation `steps`
+    times from the sorted keys 0..n-1, its time per call and a full check of
+    the last call's layout against the oracle's position functions"""
+    import torch
+    import tl_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    n = 1 << args.log2n
+    A = torch.empty(n, dtype=torch.int64, device=dev)
+    out = {}
+    for layout, b in (("bst", 1), ("btree", args.b), ("mveb", 0)):
+        times = []
+        for _ in range(max(1, args.steps)):
+            torch.arange(n, dtype=torch.int64, device=dev, out=A)
+            times.append(tl_amd.permute_involutions(layout, A, b=b, kind="u64"))
+        ranks = np.random.default_rng(7).integers(0, n, size=20_000, dtype=np.int64)
+        o = ol.oracle()
+        if layout == "mveb":
+            pos = [o.or_mveb_pos(int(x), n) for x in ranks]
+        else:
+            pos = [o.or_btree_pos(int(x), n, b) for x in ranks]
+        got = A[torch.tensor(pos, dtype=torch.int64, device=dev)].cpu().numpy()
+        out[layout] = {"b": b, "ms": round(float(np.mean(times)), 3),
+                       "runs_ms": [round(t, 3) for t in times],
+                       "layout_sample": bool(np.array_equal(got, ranks))}
+        log(f"[bench] {layout} involutions {np.mean(times):.3f} ms")
+    print(json.dumps({"workload": f"involution family, n=2^{args.log2n} uint64", "layouts": out}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -627,7 +658,7 @@ def main():
                     help="leave the batch permutations out of the headline line")
     ap.add_argument("--cpu-log2n", type=int, default=28, help="CPU baseline sample (log2 keys)")
     ap.add_argument("--cpu-iters", type=int, default=10, help="CPU baseline runs")
-    ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb"],
+    ap.add_argument("--workload", choices=["btree", "batch", "bst", "mveb", "veb", "involutions"],
                     default="btree")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--batch-perm-only", action="store_true",
@@ -653,6 +684,8 @@ def main():
         return run_batch(args, dev)
     if args.workload in ("bst", "mveb", "veb"):
         return run_layout(args, dev, args.workload)
+    if args.workload == "involutions":
+        return run_involutions(args, dev)
     n, nq, b = 1 << args.log2n, 1 << args.log2q, args.b
     rng = np.random.default_rng(4)
     log(f"[bench] generating n={n} keys and {nq} queries")
