@@ -772,7 +772,20 @@ __global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
     const uint64_t hi = min(m.count, lo + MCOPY_CHUNK);
     T *dst = (T *)m.dst;
     const T *src = (const T *)m.src;
-    for (uint64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) dst[k * m.dstride] = src[k * m.sstride];
+    // source and destination never overlap (array <-> stash), so every load
+    // of the thread is issued before its first store
+    constexpr int PER = (int)(MCOPY_CHUNK / 256);
+    T v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint64_t k = lo + threadIdx.x + (uint64_t)j * 256;
+        if (k < hi) v[j] = src[k * m.sstride];
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint64_t k = lo + threadIdx.x + (uint64_t)j * 256;
+        if (k < hi) dst[k * m.dstride] = v[j];
+    }
 }
 
 template <typename T>
