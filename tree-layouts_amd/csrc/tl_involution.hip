@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 
 namespace tl {
 
@@ -112,7 +114,11 @@ struct IPass {
     uint64_t k = 0;        // digit base / multiplier / first part of IV_REV2
     FastDiv kd;            // division by k (IV_DIGITS)
     FastDiv lend;          // division by len (a lane's partition)
+    uint64_t tab = 0;      // device table of the partners of 0 .. len-1 (0: computed per key)
 };
+
+// partners come from a table when a pass's shape repeats over partitions
+constexpr uint64_t IV_TABLE_MAX = 1u << 17;
 
 __device__ inline bool partner(const IPass &p, uint32_t kind, uint64_t i, uint64_t &j) {
     if (kind == IV_DIGITS) {
@@ -126,6 +132,25 @@ __device__ inline bool partner(const IPass &p, uint32_t kind, uint64_t i, uint64
     return j > i;
 }
 
+// the partner of i from the pass's table (an entry j <= i: no swap), or
+// computed
+__device__ inline bool partner_of(const IPass &p, uint32_t kind, uint64_t i, uint64_t &j) {
+    if (p.tab) {
+        j = ((const uint32_t *)p.tab)[i];
+        return j > i;
+    }
+    return partner(p, kind, i, j);
+}
+
+// the table of one pass shape: entry i = i's partner when it is larger, else i
+__global__ __launch_bounds__(256) void k_partner_table(IPass p, uint32_t *__restrict__ tab) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < p.len;
+         i += (uint64_t)gridDim.x * 256) {
+        uint64_t j;
+        tab[i] = (uint32_t)(partner(p, p.kind, i, j) ? j : i);
+    }
+}
+
 template <typename T, uint32_t KIND>
 __global__ __launch_bounds__(256) void k_involution(T *__restrict__ A, IPass p) {
     const uint64_t total = p.parts * p.len;
@@ -134,7 +159,7 @@ __global__ __launch_bounds__(256) void k_involution(T *__restrict__ A, IPass p) 
         const uint64_t q = p.parts == 1 ? 0 : fdiv(g, p.lend);
         const uint64_t i = g - q * p.len;
         uint64_t j;
-        if (!partner(p, KIND, i, j)) continue;
+        if (!partner_of(p, KIND, i, j)) continue;
         const int64_t o = p.base + (int64_t)(q * p.stride);
         const T a = A[o + (int64_t)i], b = A[o + (int64_t)j];
         A[o + (int64_t)i] = b;
@@ -162,7 +187,7 @@ __global__ __launch_bounds__(256) void k_involution_lds(T *__restrict__ A, int64
                 const uint64_t pq = p.parts == 1 ? 0 : fdiv(g, p.lend);
                 const uint64_t i = g - pq * p.len;
                 uint64_t j;
-                if (!partner(p, p.kind, i, j)) continue;
+                if (!partner_of(p, p.kind, i, j)) continue;
                 const int64_t o = p.base + (int64_t)(pq * p.stride);
                 const T x = s[o + (int64_t)i];
                 s[o + (int64_t)i] = s[o + (int64_t)j];
@@ -494,16 +519,58 @@ static tl_status width(tl_dtype dt, uint32_t &W) {
     return TL_OK;
 }
 
-static tl_status run_timed(const IPlan &pl, void *dev_A, uint32_t W, void *stream, float *ms,
+static tl_status run_timed(IPlan pl, void *dev_A, uint32_t W, void *stream, float *ms,
                            uint64_t bst_perfect = 0, tl_dtype dt = TL_U64) {
     Ctx *c;
     TL_TRY(get_ctx(&c));
     hipStream_t s = (hipStream_t)stream;
+    // partner tables: one per pass shape (kind, digits, length, base or
+    // multiplier) that repeats over partitions or runs in LDS; reversals
+    // stay computed
+    std::map<std::tuple<uint32_t, uint32_t, uint64_t, uint64_t>, uint64_t> shapes;
+    std::vector<IPass> reps;
+    uint64_t entries = 0;
+    auto want = [&](IPass &p) {
+        if (p.kind == IV_REV2 || p.len > IV_TABLE_MAX) return;
+        const auto key = std::make_tuple(p.kind, p.nd, p.len, p.k);
+        auto it = shapes.find(key);
+        if (it == shapes.end()) {
+            it = shapes.emplace(key, entries).first;
+            reps.push_back(p);
+            reps.back().tab = entries;
+            entries += p.len;
+        }
+        p.tab = it->second + 1;   // entry offset + 1 until the tables are placed
+    };
+    for (IStep &st : pl.steps) {
+        if (st.lds)
+            for (IPass &p : st.prog) want(p);
+        else if (st.pass.parts > 1)
+            want(st.pass);
+    }
+    uint32_t *tabs = nullptr;
+    if (entries) {
+        TL_TRY(scratch(c, 3, entries * 4, (void **)&tabs));
+        auto place = [&](IPass &p) {
+            if (p.tab) p.tab = (uint64_t)(tabs + (p.tab - 1));
+        };
+        for (IStep &st : pl.steps) {
+            place(st.pass);
+            for (IPass &p : st.prog) place(p);
+        }
+    }
     IPass *dev_table;
     std::vector<size_t> at;
     TL_TRY(upload_groups(c, pl, s, &dev_table, at));
     Timer tm;
     TL_TRY(tm.start(s, ms));
+    for (const IPass &r : reps) {
+        IPass q = r;
+        q.tab = 0;
+        const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((r.len + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_partner_table, dim3(g), dim3(256), 0, s, q, tabs + r.tab);
+        TL_LAUNCH_CHECK();
+    }
     TL_TRY(W == 4 ? run_iplan<uint32_t>(c, pl, (uint32_t *)dev_A, dev_table, at, s)
                   : run_iplan<uint64_t>(c, pl, (uint64_t *)dev_A, dev_table, at, s));
     if (bst_perfect > 1) TL_TRY(tl_permute_bst(dev_A, bst_perfect, dt, stream, nullptr));
