@@ -1080,10 +1080,11 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 MCopy d{};
                 const uint64_t Ap = A + op.p[0] * W, St = stash + op.p[4] * W;
                 if (op.p[3])
-                    d = MCopy{St, Ap, 1, op.p[1], op.p[2]};
+                    d = MCopy{St, Ap, 1, op.p[1], op.p[2], W};
                 else
-                    d = MCopy{Ap, St, op.p[1], 1, op.p[2]};
-                add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[2] + 2047) / 2048, 0);
+                    d = MCopy{Ap, St, op.p[1], 1, op.p[2], W};
+                // copies of both key widths share launches (key width 0)
+                add(ga, MK_COPY, 0, 0, 256, &d, sizeof d, (op.p[2] + 2047) / 2048, 0);
                 break;
             }
             case OP_PART_ROOTS: {
@@ -1094,8 +1095,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
             }
             case OP_ROTATE: {   // the stash copy, then (a later launch) the lane walk
                 if (!npart[li]) {
-                    const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3]};
-                    add(ga, MK_COPY, W, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
+                    const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3], W};
+                    add(ga, MK_COPY, 0, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
                 } else {
                     const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3]};
                     const uint32_t wb = multi_walk_block();
@@ -1155,8 +1156,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 for (uint32_t t = t0; t < d; t++) {
                     const uint64_t cnt = 1ull << (t - K);
                     const MCopy c{stash + rowend_slots(t, K, t0) * W, A + (1ull << t) * W, 1, 1,
-                                  cnt};
-                    add(ga, MK_COPY, W, 0, 256, &c, sizeof c, (cnt + 2047) / 2048, 0);
+                                  cnt, W};
+                    add(ga, MK_COPY, 0, 0, 256, &c, sizeof c, (cnt + 2047) / 2048, 0);
                 }
                 break;
             }

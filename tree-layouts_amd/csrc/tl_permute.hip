@@ -764,11 +764,8 @@ __device__ inline uint32_t find_desc(const uint64_t *__restrict__ pre, uint32_t 
 constexpr uint64_t MCOPY_CHUNK = 2048;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
-                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
-    const uint32_t i = find_desc(pre, nd, blockIdx.x);
-    const MCopy m = d[i];
-    const uint64_t lo = (blockIdx.x - pre[i]) * MCOPY_CHUNK;
+__device__ inline void multi_copy_body(const MCopy &m, uint64_t part) {
+    const uint64_t lo = part * MCOPY_CHUNK;
     const uint64_t hi = min(m.count, lo + MCOPY_CHUNK);
     T *dst = (T *)m.dst;
     const T *src = (const T *)m.src;
@@ -786,6 +783,18 @@ __global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
         const uint64_t k = lo + threadIdx.x + (uint64_t)j * 256;
         if (k < hi) dst[k * m.dstride] = v[j];
     }
+}
+
+// the copies of arrays of both key widths in one launch (the width is per
+// descriptor)
+__global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
+                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MCopy m = d[i];
+    if (m.w == 4)
+        multi_copy_body<uint32_t>(m, blockIdx.x - pre[i]);
+    else
+        multi_copy_body<uint64_t>(m, blockIdx.x - pre[i]);
 }
 
 template <typename T>
@@ -926,7 +935,7 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     const uint64_t *pre = (const uint64_t *)m.pre;
     switch (m.kind) {
     case MK_COPY:
-        hipLaunchKernelGGL(k_multi_copy<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_multi_copy, dim3((uint32_t)m.tasks), dim3(256), 0, s,
                            (const MCopy *)m.desc, pre, m.nd);
         break;
     case MK_LDS:

@@ -146,8 +146,8 @@ tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W,
 //      independent arrays (the uneven batch runs a plan's k-th op of every
 //      array together).  Descriptors hold absolute device addresses; `pre`
 //      is the prefix sum of each descriptor's workgroup (or move) count.
-struct MCopy {   // dst[i*dstride] = src[i*sstride], i < count
-    uint64_t dst, src, dstride, sstride, count;
+struct MCopy {   // dst[i*dstride] = src[i*sstride], i < count, keys of w bytes
+    uint64_t dst, src, dstride, sstride, count, w;
 };
 struct MLds {    // one OP_LDS of one array
     uint64_t A, runs, nitems;
