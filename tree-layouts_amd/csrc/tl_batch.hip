@@ -1098,9 +1098,10 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                     const MCopy d{stash, A + op.p[0] * W, 1, 1, op.p[3], W};
                     add(ga, MK_COPY, 0, 0, 256, &d, sizeof d, (op.p[3] + 2047) / 2048, 0);
                 } else {
-                    const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3]};
+                    const MWalk w{A + op.p[0] * W, stash, op.p[1], op.p[2], op.p[3], W};
                     const uint32_t wb = multi_walk_block();
-                    add(ga, MK_WALK, W, 0, wb, &w, sizeof w, (op.p[3] + wb - 1) / wb, 0);
+                    // walks of both key widths share launches (key width 0)
+                    add(ga, MK_WALK, 0, 0, wb, &w, sizeof w, (op.p[3] + wb - 1) / wb, 0);
                 }
                 break;
             }

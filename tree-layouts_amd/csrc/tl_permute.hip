@@ -909,13 +909,19 @@ __global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict_
     }
 }
 
-template <typename T, int BS = 256, int K = 4>
+// the walks of arrays of both key widths in one launch (the width is per
+// descriptor)
+template <int BS = 256, int K = 4>
 __global__ __launch_bounds__(BS) void k_multi_walk(const MWalk *__restrict__ d,
                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MWalk m = d[i];
     const uint64_t j = (blockIdx.x - pre[i]) * BS + threadIdx.x;
-    if (j < m.w) walk_lane<T, K>((T *)m.A, m.L, m.c, m.w, (const T *)m.stash, j);
+    if (j >= m.w) return;
+    if (m.W == 4)
+        walk_lane<uint32_t, K>((uint32_t *)m.A, m.L, m.c, m.w, (const uint32_t *)m.stash, j);
+    else
+        walk_lane<uint64_t, K>((uint64_t *)m.A, m.L, m.c, m.w, (const uint64_t *)m.stash, j);
 }
 
 // steps of a multi-array rotation lane walk with their loads in flight
@@ -1005,19 +1011,19 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     }
     case MK_WALK:
         if (m.threads == 1024 && multi_walk_depth() == 16)
-            hipLaunchKernelGGL((k_multi_walk<T, 1024, 16>), dim3((uint32_t)m.tasks), dim3(1024), 0,
+            hipLaunchKernelGGL((k_multi_walk<1024, 16>), dim3((uint32_t)m.tasks), dim3(1024), 0,
                                s, (const MWalk *)m.desc, pre, m.nd);
         else if (m.threads == 1024 && multi_walk_depth() == 8)
-            hipLaunchKernelGGL((k_multi_walk<T, 1024, 8>), dim3((uint32_t)m.tasks), dim3(1024), 0,
+            hipLaunchKernelGGL((k_multi_walk<1024, 8>), dim3((uint32_t)m.tasks), dim3(1024), 0,
                                s, (const MWalk *)m.desc, pre, m.nd);
         else if (m.threads == 1024)
-            hipLaunchKernelGGL((k_multi_walk<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024), 0, s,
+            hipLaunchKernelGGL((k_multi_walk<1024>), dim3((uint32_t)m.tasks), dim3(1024), 0, s,
                                (const MWalk *)m.desc, pre, m.nd);
         else if (multi_walk_depth() == 16)
-            hipLaunchKernelGGL((k_multi_walk<T, 256, 16>), dim3((uint32_t)m.tasks), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_multi_walk<256, 16>), dim3((uint32_t)m.tasks), dim3(256), 0, s,
                                (const MWalk *)m.desc, pre, m.nd);
         else
-            hipLaunchKernelGGL(k_multi_walk<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+            hipLaunchKernelGGL(k_multi_walk<>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
                                (const MWalk *)m.desc, pre, m.nd);
         break;
     case MK_PROOTS:

@@ -167,8 +167,8 @@ struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
     uint32_t D, t0, d, levels;
     uint64_t stash;   // BITREV_SHIFT levels: the array's level heads
 };
-struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before)
-    uint64_t A, stash, L, c, w;
+struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before); keys of W bytes
+    uint64_t A, stash, L, c, w, W;
 };
 struct MPRoots {   // one OP_PART_ROOTS of one array (stash: its first root)
     uint64_t A, stash, base, stride, count;
