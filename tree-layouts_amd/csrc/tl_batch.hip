@@ -1131,11 +1131,12 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 for (size_t q = 0; q < k; q++)
                     if (L.plan.ops[q].type == OP_CHAINED)
                         off += ((L.plan.ops[q].p[0] + 15) / 16) * 16 + 16;
-                const MChain d{A, (uint64_t)B.d_pflags + L.flags_at + off * 4, stash, op.ms};
+                const MChain d{A, (uint64_t)B.d_pflags + L.flags_at + off * 4, stash, op.ms, W};
                 const uint32_t mode = op.ms.kind == MV_ROW_SPREAD      ? (uint32_t)CH_ROW_SPREAD
                                       : op.ms.kind == MV_SPARSE_SPREAD ? (uint32_t)CH_SPARSE_SPREAD
                                                                        : (uint32_t)CH_FORWARD;
-                add(ga, MK_CHAIN, W, mode, 256, &d, sizeof d, op.p[0], 0);
+                // chained moves of both key widths share launches (key width 0)
+                add(ga, MK_CHAIN, 0, mode, 256, &d, sizeof d, op.p[0], 0);
                 break;
             }
             case OP_BITREV: {   // plain, or phase one of a kept tree (the same at base 0)

@@ -836,8 +836,10 @@ __global__ __launch_bounds__(NT) void k_multi_stream_fused(const MStream *__rest
 }
 
 // moves of many chained ops, numbered op after op: a move waits only for
-// lower-numbered moves of its own op, which hold lower tickets
-template <typename T, int PER, int MODE>
+// lower-numbered moves of its own op, which hold lower tickets.  The ops'
+// arrays may have either key width (per descriptor; a workgroup's move
+// belongs to one op, so its branch is uniform)
+template <int PER, int MODE>
 __global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict__ d,
                                                        const uint64_t *__restrict__ pre, uint32_t nd,
                                                        uint64_t total, unsigned long long *counter,
@@ -862,8 +864,14 @@ __global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict_
                 have = want;
             }
         }
-        chained_move<T, PER, MODE>((T *)m.A, m.ms, t - pre[i], (uint32_t *)m.flags, err, limit,
-                                   (T *)m.stash, s_slot);
+        if (m.W == 4)
+            chained_move<uint32_t, PER, MODE>((uint32_t *)m.A, m.ms, t - pre[i],
+                                              (uint32_t *)m.flags, err, limit,
+                                              (uint32_t *)m.stash, s_slot);
+        else
+            chained_move<uint64_t, PER, MODE>((uint64_t *)m.A, m.ms, t - pre[i],
+                                              (uint32_t *)m.flags, err, limit,
+                                              (uint64_t *)m.stash, s_slot);
     }
 }
 
@@ -974,15 +982,15 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(m.tasks, (uint64_t)c->num_cus * 8);
         unsigned long long *ctr = (unsigned long long *)m.counter;
         if (m.param == CH_ROW_SPREAD)
-            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_ROW_SPREAD>), dim3(grid),
+            hipLaunchKernelGGL((k_multi_chained<CHAINED_PER, CH_ROW_SPREAD>), dim3(grid),
                                dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
                                c->err_flag, c->wait_ticks);
         else if (m.param == CH_SPARSE_SPREAD)
-            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_SPARSE_SPREAD>), dim3(grid),
+            hipLaunchKernelGGL((k_multi_chained<CHAINED_PER, CH_SPARSE_SPREAD>), dim3(grid),
                                dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
                                c->err_flag, c->wait_ticks);
         else
-            hipLaunchKernelGGL((k_multi_chained<T, CHAINED_PER, CH_FORWARD>), dim3(grid),
+            hipLaunchKernelGGL((k_multi_chained<CHAINED_PER, CH_FORWARD>), dim3(grid),
                                dim3(256), 0, s, (const MChain *)m.desc, pre, m.nd, m.tasks, ctr,
                                c->err_flag, c->wait_ticks);
         break;

@@ -157,9 +157,10 @@ struct MStream {   // one OP_SPARSE_STREAM of one array
     uint64_t A, runs;
     uint32_t nruns, r, l, inv, fuse, pad;
 };
-struct MChain {    // one OP_CHAINED of one array
+struct MChain {    // one OP_CHAINED of one array, keys of W bytes
     uint64_t A, flags, stash;
     MoveSpec ms;
+    uint64_t W;
 };
 struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
     uint64_t A;
