@@ -1110,18 +1110,19 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 const MLds d{A, runs, op.nitems, (uint32_t)op.nruns, (uint32_t)per, (uint32_t)n,
                              (uint32_t)op.p[1], (uint32_t)op.p[2], (uint32_t)op.p[3],
                              (uint32_t)((0xFFFFFFFFull + n) / n), (uint32_t)op.p[5],
-                             (uint32_t)op.p[6], 0};
+                             (uint32_t)op.p[6], W};
                 const uint32_t threads = (per * n >= 8192) ? 512 : 256;
-                add(ga, MK_LDS, W, 0, threads, &d, sizeof d, (op.nitems + per - 1) / per,
+                // both key widths share launches (key width 0; LDS for the largest)
+                add(ga, MK_LDS, 0, 0, threads, &d, sizeof d, (op.nitems + per - 1) / per,
                     per * n * W + (per > 1 ? n * 4 : 0));
                 break;
             }
             case OP_SPARSE_STREAM: {
                 const uint64_t r = op.p[0], l = op.p[1];
                 const MStream d{A, runs, (uint32_t)op.nruns, (uint32_t)r, (uint32_t)l,
-                                (uint32_t)op.p[2], (uint32_t)op.p[3], 0};
+                                (uint32_t)op.p[2], (uint32_t)op.p[3], W};
                 const bool fused = op.p[3] != 0;
-                add(ga, fused ? MK_FUSED : MK_STREAM, W, 0, stream_threads(B), &d, sizeof d,
+                add(ga, fused ? MK_FUSED : MK_STREAM, 0, 0, stream_threads(B), &d, sizeof d,
                     op.nitems, fused ? ((r * W + 15) & ~15ull) + l * 4 : r * W);
                 break;
             }

@@ -797,42 +797,65 @@ __global__ __launch_bounds__(256) void k_multi_copy(const MCopy *__restrict__ d,
         multi_copy_body<uint64_t>(m, blockIdx.x - pre[i]);
 }
 
-template <typename T>
+// in-LDS layouts of arrays of both key widths in one launch (the width is
+// per descriptor; the dynamic LDS is sized for the largest)
 __global__ __launch_bounds__(512) void k_multi_lds(const MLds *__restrict__ d,
                                                    const uint64_t *__restrict__ pre, uint32_t nd) {
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MLds m = d[i];
-    lds_body<T>((T *)m.A, (const uint64_t *)m.runs, m.nruns, m.nitems, m.per, m.n, m.kind, m.b,
-                m.rows, m.magic, m.inv, blockIdx.x - pre[i], m.shift);
+    if (m.W == 4)
+        lds_body<uint32_t>((uint32_t *)m.A, (const uint64_t *)m.runs, m.nruns, m.nitems, m.per,
+                           m.n, m.kind, m.b, m.rows, m.magic, m.inv, blockIdx.x - pre[i],
+                           m.shift);
+    else
+        lds_body<uint64_t>((uint64_t *)m.A, (const uint64_t *)m.runs, m.nruns, m.nitems, m.per,
+                           m.n, m.kind, m.b, m.rows, m.magic, m.inv, blockIdx.x - pre[i],
+                           m.shift);
 }
 
-// NT threads per partition (the stream's batches are NT * 8 keys; any batch
-// size keeps every batch read before it is written)
 template <typename T, uint32_t NT>
-__global__ __launch_bounds__(NT) void k_multi_stream(const MStream *__restrict__ d,
-                                                     const uint64_t *__restrict__ pre,
-                                                     uint32_t nd) {
-    const uint32_t i = find_desc(pre, nd, blockIdx.x);
-    const MStream m = d[i];
-    T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, blockIdx.x - pre[i]);
+__device__ inline void multi_stream_body(const MStream &m, uint64_t part) {
+    T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, part);
     if (m.fuse)
         stream_fused_body<T, NT>(a, m.r, m.l, m.inv, m.fuse);
     else
         stream_body<T>(a, m.r, m.l, m.inv);
 }
 
+// NT threads per partition (the stream's batches are NT * 8 keys; any batch
+// size keeps every batch read before it is written); either key width, per
+// descriptor
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void k_multi_stream(const MStream *__restrict__ d,
+                                                     const uint64_t *__restrict__ pre,
+                                                     uint32_t nd) {
+    const uint32_t i = find_desc(pre, nd, blockIdx.x);
+    const MStream m = d[i];
+    if (m.W == 4)
+        multi_stream_body<uint32_t, NT>(m, blockIdx.x - pre[i]);
+    else
+        multi_stream_body<uint64_t, NT>(m, blockIdx.x - pre[i]);
+}
+
 // fused streamed gathers of many arrays, NT threads per partition: a
 // partition's workgroup finishes NT / 256 times sooner than with 256 threads,
 // so a launch whose partitions fill the device a little over a whole number
 // of times loses less to its last, partial round
-template <typename T, uint32_t NT>
+template <uint32_t NT>
 __global__ __launch_bounds__(NT) void k_multi_stream_fused(const MStream *__restrict__ d,
                                                            const uint64_t *__restrict__ pre,
                                                            uint32_t nd) {
     const uint32_t i = find_desc(pre, nd, blockIdx.x);
     const MStream m = d[i];
-    T *a = (T *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, blockIdx.x - pre[i]);
-    stream_fused_body<T, NT>(a, m.r, m.l, m.inv, m.fuse);
+    const uint64_t part = blockIdx.x - pre[i];
+    if (m.W == 4)
+        stream_fused_body<uint32_t, NT>(
+            (uint32_t *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, part), m.r, m.l,
+            m.inv, m.fuse);
+    else
+        stream_fused_body<uint64_t, NT>(
+            (uint64_t *)m.A + run_offset((const uint64_t *)m.runs, m.nruns, part), m.r, m.l,
+            m.inv, m.fuse);
 }
 
 // moves of many chained ops, numbered op after op: a move waits only for
@@ -953,29 +976,29 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
                            (const MCopy *)m.desc, pre, m.nd);
         break;
     case MK_LDS:
-        hipLaunchKernelGGL(k_multi_lds<T>, dim3((uint32_t)m.tasks), dim3(m.threads), m.lds, s,
+        hipLaunchKernelGGL(k_multi_lds, dim3((uint32_t)m.tasks), dim3(m.threads), m.lds, s,
                            (const MLds *)m.desc, pre, m.nd);
         break;
     case MK_STREAM:
         if (m.threads == 1024)
-            hipLaunchKernelGGL((k_multi_stream<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024),
+            hipLaunchKernelGGL((k_multi_stream<1024>), dim3((uint32_t)m.tasks), dim3(1024),
                                m.lds, s, (const MStream *)m.desc, pre, m.nd);
         else if (m.threads == 512)
-            hipLaunchKernelGGL((k_multi_stream<T, 512>), dim3((uint32_t)m.tasks), dim3(512), m.lds,
+            hipLaunchKernelGGL((k_multi_stream<512>), dim3((uint32_t)m.tasks), dim3(512), m.lds,
                                s, (const MStream *)m.desc, pre, m.nd);
         else
-            hipLaunchKernelGGL((k_multi_stream<T, 256>), dim3((uint32_t)m.tasks), dim3(256), m.lds,
+            hipLaunchKernelGGL((k_multi_stream<256>), dim3((uint32_t)m.tasks), dim3(256), m.lds,
                                s, (const MStream *)m.desc, pre, m.nd);
         break;
     case MK_FUSED:
         if (m.threads == 1024)
-            hipLaunchKernelGGL((k_multi_stream_fused<T, 1024>), dim3((uint32_t)m.tasks), dim3(1024),
+            hipLaunchKernelGGL((k_multi_stream_fused<1024>), dim3((uint32_t)m.tasks), dim3(1024),
                                m.lds, s, (const MStream *)m.desc, pre, m.nd);
         else if (m.threads == 512)
-            hipLaunchKernelGGL((k_multi_stream_fused<T, 512>), dim3((uint32_t)m.tasks), dim3(512),
+            hipLaunchKernelGGL((k_multi_stream_fused<512>), dim3((uint32_t)m.tasks), dim3(512),
                                m.lds, s, (const MStream *)m.desc, pre, m.nd);
         else
-            hipLaunchKernelGGL((k_multi_stream_fused<T, 256>), dim3((uint32_t)m.tasks), dim3(256),
+            hipLaunchKernelGGL((k_multi_stream_fused<256>), dim3((uint32_t)m.tasks), dim3(256),
                                m.lds, s, (const MStream *)m.desc, pre, m.nd);
         break;
     case MK_CHAIN: {

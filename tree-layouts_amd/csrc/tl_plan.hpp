@@ -149,13 +149,13 @@ tl_status lds_items_launch(const uint64_t *dev_desc, uint32_t count, uint32_t W,
 struct MCopy {   // dst[i*dstride] = src[i*sstride], i < count, keys of w bytes
     uint64_t dst, src, dstride, sstride, count, w;
 };
-struct MLds {    // one OP_LDS of one array
+struct MLds {    // one OP_LDS of one array, keys of W bytes
     uint64_t A, runs, nitems;
-    uint32_t nruns, per, n, kind, b, rows, magic, inv, shift, pad;
+    uint32_t nruns, per, n, kind, b, rows, magic, inv, shift, W;
 };
-struct MStream {   // one OP_SPARSE_STREAM of one array
+struct MStream {   // one OP_SPARSE_STREAM of one array, keys of W bytes
     uint64_t A, runs;
-    uint32_t nruns, r, l, inv, fuse, pad;
+    uint32_t nruns, r, l, inv, fuse, W;
 };
 struct MChain {    // one OP_CHAINED of one array, keys of W bytes
     uint64_t A, flags, stash;
