@@ -1089,8 +1089,8 @@ static tl_status build_waves(Batch &B, Ctx *c) {
             }
             case OP_PART_ROOTS: {
                 const MPRoots d{A, stash + op.p[4] * W, op.p[0], op.p[1], op.p[2],
-                                (uint32_t)op.p[3], (uint32_t)(op.p[5] == 2), (uint32_t)op.p[6], 0};
-                add(ga, MK_PROOTS, W, 0, 256, &d, sizeof d, (op.p[2] * op.p[3] + 255) / 256, 0);
+                                (uint32_t)op.p[3], (uint32_t)(op.p[5] == 2), (uint32_t)op.p[6], W};
+                add(ga, MK_PROOTS, 0, 0, 256, &d, sizeof d, (op.p[2] * op.p[3] + 255) / 256, 0);
                 break;
             }
             case OP_ROTATE: {   // the stash copy, then (a later launch) the lane walk
@@ -1141,15 +1141,16 @@ static tl_status build_waves(Batch &B, Ctx *c) {
                 break;
             }
             case OP_BITREV: {   // plain, or phase one of a kept tree (the same at base 0)
-                const MBitrev d{A, (int64_t)op.p[0], (uint32_t)op.p[1], 0, 0, 0, 0};
-                add(ga, MK_BITREV, W, (uint32_t)op.p[2], 256, &d, sizeof d,
+                const MBitrev d{A, (int64_t)op.p[0], (uint32_t)op.p[1], 0, 0, 0, 0, W};
+                // both key widths share launches (key width 0)
+                add(ga, MK_BITREV, 0, (uint32_t)op.p[2], 256, &d, sizeof d,
                     1ull << (op.p[1] - 2 * op.p[2]), 0);
                 break;
             }
             case OP_BITREV_LEVELS: {   // plain, or a kept tree's levels shifted
                 const bool shift = op.p[3] == BITREV_SHIFT;
-                const MBitrev d{A, 0, 0, (uint32_t)op.p[0], (uint32_t)op.p[1], 1, stash};
-                add(ga, MK_BITREV, W, (uint32_t)op.p[2] | (shift ? 1u << 8 : 0u), 256, &d,
+                const MBitrev d{A, 0, 0, (uint32_t)op.p[0], (uint32_t)op.p[1], 1, stash, W};
+                add(ga, MK_BITREV, 0, (uint32_t)op.p[2] | (shift ? 1u << 8 : 0u), 256, &d,
                     sizeof d, (1ull << (op.p[1] - op.p[0])) - 1, 0);
                 break;
             }

@@ -63,13 +63,22 @@ __device__ inline uint64_t rowend_slot(uint64_t Y, uint32_t t0) {
 // A tile pair reads all its slots before it writes any, and every key a tile
 // writes over is one of its own slots' or a row end put in S, so tiles need
 // no order among themselves.
+// The LDS of a tile pair: two tiles of 2^K x (2^K + 1) keys of up to 8 bytes
+// and two row-end vectors, one buffer per tile size, so a kernel that handles
+// both key widths holds it once.
+template <int K>
+__device__ inline uint64_t *tile_lds() {
+    __shared__ uint64_t buf[2 * (1 << K) * ((1 << K) + 1) + 2 * (1 << K)];
+    return buf;
+}
+
 template <typename T, int K, int FOLD = 0>
 __device__ inline void bitrev_tile(T *__restrict__ A, int64_t base, uint32_t D, uint64_t t,
                                    T *__restrict__ S = nullptr, uint32_t t0 = 0) {
     constexpr int R = 1 << K;
     constexpr int E = R * R / 256;   // elements per thread and tile
-    __shared__ T sA[R][R + 1];
-    __shared__ T sB[R][R + 1];
+    T (*sA)[R + 1] = reinterpret_cast<T (*)[R + 1]>(tile_lds<K>());
+    T (*sB)[R + 1] = sA + R;
     const uint32_t MB = D - 2 * K;
     constexpr uint32_t row_log2 = (uint32_t)K + (sizeof(T) == 8 ? 3u : 2u);
     const uint64_t m = bitrev_tile_order(t, MB, 21u - row_log2);
@@ -132,9 +141,10 @@ __device__ inline void bitrev_tile_shift(T *__restrict__ A, int64_t base, uint32
                                          const T *__restrict__ H, bool first, bool last) {
     constexpr int R = 1 << K;
     constexpr int E = R * R / 256;
-    __shared__ T sA[R][R + 1];
-    __shared__ T sB[R][R + 1];
-    __shared__ T nA[R], nB[R];
+    T (*sA)[R + 1] = reinterpret_cast<T (*)[R + 1]>(tile_lds<K>());
+    T (*sB)[R + 1] = sA + R;
+    T *nA = reinterpret_cast<T *>(sB + R);
+    T *nB = nA + R;
     const uint32_t MB = D - 2 * K;
     constexpr uint32_t row_log2 = (uint32_t)K + (sizeof(T) == 8 ? 3u : 2u);
     const uint64_t m = bitrev_tile_order(t, MB, 21u - row_log2);

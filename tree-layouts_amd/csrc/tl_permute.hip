@@ -900,14 +900,7 @@ __global__ __launch_bounds__(256) void k_multi_chained(const MChain *__restrict_
 
 // OP_PART_ROOTS of many arrays: 256 roots per workgroup
 template <typename T>
-__global__ __launch_bounds__(256) void k_multi_proots(const MPRoots *__restrict__ d,
-                                                      const uint64_t *__restrict__ pre,
-                                                      uint32_t nd) {
-    const uint64_t t = blockIdx.x;
-    const uint32_t i = find_desc(pre, nd, t);
-    const MPRoots m = d[i];
-    const uint64_t g = (t - pre[i]) * 256 + threadIdx.x;
-    if (g >= m.count * m.rp) return;
+__device__ inline void multi_proots_body(const MPRoots &m, uint64_t g) {
     const uint64_t blk = g / m.rp;
     const uint32_t a = (uint32_t)(g - blk * m.rp);
     T *slot = (T *)m.A + m.base + blk * m.stride + mveb_pos32(a, m.rp, m.classic != 0);
@@ -918,17 +911,25 @@ __global__ __launch_bounds__(256) void k_multi_proots(const MPRoots *__restrict_
         *slot = *st;
 }
 
-// SHIFT = 1: every descriptor is the levels of a BST kept one place right,
-// written one slot left (BITREV_SHIFT)
-template <typename T, int K, int SHIFT = 0>
-__global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict__ d,
+// either key width, per descriptor
+__global__ __launch_bounds__(256) void k_multi_proots(const MPRoots *__restrict__ d,
                                                       const uint64_t *__restrict__ pre,
                                                       uint32_t nd) {
-    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
-    if (t >= pre[nd]) return;
+    const uint64_t t = blockIdx.x;
     const uint32_t i = find_desc(pre, nd, t);
-    const MBitrev m = d[i];
-    const uint64_t g = t - pre[i];
+    const MPRoots m = d[i];
+    const uint64_t g = (t - pre[i]) * 256 + threadIdx.x;
+    if (g >= m.count * m.rp) return;
+    if (m.W == 4)
+        multi_proots_body<uint32_t>(m, g);
+    else
+        multi_proots_body<uint64_t>(m, g);
+}
+
+// SHIFT = 1: every descriptor is the levels of a BST kept one place right,
+// written one slot left (BITREV_SHIFT)
+template <typename T, int K, int SHIFT>
+__device__ inline void multi_bitrev_body(const MBitrev &m, uint64_t g) {
     if constexpr (SHIFT == 1) {
         level_tile<T, K, 1>((T *)m.A, m.t0, m.d, g, (T *)m.stash);
     } else if (m.levels) {
@@ -938,6 +939,21 @@ __global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict_
     } else {
         bitrev_tile<T, K>((T *)m.A, m.base, m.D, g);
     }
+}
+
+// either key width, per descriptor (the tile buffer is the 8-byte one)
+template <int K, int SHIFT = 0>
+__global__ __launch_bounds__(256) void k_multi_bitrev(const MBitrev *__restrict__ d,
+                                                      const uint64_t *__restrict__ pre,
+                                                      uint32_t nd) {
+    const uint64_t t = (uint64_t)blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+    if (t >= pre[nd]) return;
+    const uint32_t i = find_desc(pre, nd, t);
+    const MBitrev m = d[i];
+    if (m.W == 4)
+        multi_bitrev_body<uint32_t, K, SHIFT>(m, t - pre[i]);
+    else
+        multi_bitrev_body<uint64_t, K, SHIFT>(m, t - pre[i]);
 }
 
 // the walks of arrays of both key widths in one launch (the width is per
@@ -1025,17 +1041,17 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
         const bool shift = (m.param >> 8) == 1;
         if ((m.param & 0xff) == 6) {
             if (shift)
-                hipLaunchKernelGGL((k_multi_bitrev<T, 6, 1>), dim3(gx, gy), dim3(256), 0, s,
+                hipLaunchKernelGGL((k_multi_bitrev<6, 1>), dim3(gx, gy), dim3(256), 0, s,
                                    (const MBitrev *)m.desc, pre, m.nd);
             else
-                hipLaunchKernelGGL((k_multi_bitrev<T, 6>), dim3(gx, gy), dim3(256), 0, s,
+                hipLaunchKernelGGL((k_multi_bitrev<6>), dim3(gx, gy), dim3(256), 0, s,
                                    (const MBitrev *)m.desc, pre, m.nd);
         } else {
             if (shift)
-                hipLaunchKernelGGL((k_multi_bitrev<T, 5, 1>), dim3(gx, gy), dim3(256), 0, s,
+                hipLaunchKernelGGL((k_multi_bitrev<5, 1>), dim3(gx, gy), dim3(256), 0, s,
                                    (const MBitrev *)m.desc, pre, m.nd);
             else
-                hipLaunchKernelGGL((k_multi_bitrev<T, 5>), dim3(gx, gy), dim3(256), 0, s,
+                hipLaunchKernelGGL((k_multi_bitrev<5>), dim3(gx, gy), dim3(256), 0, s,
                                    (const MBitrev *)m.desc, pre, m.nd);
         }
         break;
@@ -1058,7 +1074,7 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
                                (const MWalk *)m.desc, pre, m.nd);
         break;
     case MK_PROOTS:
-        hipLaunchKernelGGL(k_multi_proots<T>, dim3((uint32_t)m.tasks), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_multi_proots, dim3((uint32_t)m.tasks), dim3(256), 0, s,
                            (const MPRoots *)m.desc, pre, m.nd);
         break;
     default: set_error("multi_launch: unknown kind"); return TL_EINVAL;

@@ -167,13 +167,14 @@ struct MBitrev {   // OP_BITREV (levels = 0) or OP_BITREV_LEVELS (levels = 1)
     int64_t base;
     uint32_t D, t0, d, levels;
     uint64_t stash;   // BITREV_SHIFT levels: the array's level heads
+    uint64_t W;       // key width in bytes
 };
 struct MWalk {     // the lane walk of one OP_ROTATE (its stash copied before); keys of W bytes
     uint64_t A, stash, L, c, w, W;
 };
-struct MPRoots {   // one OP_PART_ROOTS of one array (stash: its first root)
+struct MPRoots {   // one OP_PART_ROOTS of one array (stash: its first root), keys of W bytes
     uint64_t A, stash, base, stride, count;
-    uint32_t rp, classic, inv, pad;
+    uint32_t rp, classic, inv, W;
 };
 
 enum MultiKind : uint32_t {
