@@ -982,8 +982,7 @@ static int multi_walk_depth() {
     return (v == 4 || v == 8) ? v : 16;
 }
 
-template <typename T>
-static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
+static tl_status multi_launch_body(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     if (!m.nd || !m.tasks) return TL_OK;
     const uint64_t *pre = (const uint64_t *)m.pre;
     switch (m.kind) {
@@ -1083,8 +1082,10 @@ static tl_status multi_launch_t(Ctx *c, const MultiLaunch &m, hipStream_t s) {
     return TL_OK;
 }
 
+// every multi-array kernel takes the key width from its descriptors, so
+// arrays of both widths share a launch
 tl_status multi_launch(Ctx *c, const MultiLaunch &m, hipStream_t s) {
-    return m.W == 4 ? multi_launch_t<uint32_t>(c, m, s) : multi_launch_t<uint64_t>(c, m, s);
+    return multi_launch_body(c, m, s);
 }
 
 // one workgroup waits for a flag that nothing sets, bounded by `limit`

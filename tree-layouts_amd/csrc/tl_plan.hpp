@@ -189,7 +189,8 @@ enum MultiKind : uint32_t {
 };
 
 struct MultiLaunch {
-    uint32_t kind, W, param;        // param: chained mode, or bitrev K
+    uint32_t kind, W, param;        // W: 0 (descriptors carry their key width); param:
+                                    // chained mode, or bitrev K
     uint32_t nd;                    // descriptors
     uint64_t desc, pre;             // device addresses of descriptors and prefix sums
     uint64_t tasks;                 // workgroups (chained: moves)
