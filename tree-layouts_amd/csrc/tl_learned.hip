@@ -138,7 +138,8 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(const K *__restrict__ keys,
 }
 
 // one workgroup per leaf: least squares of (rank - base_rank) on the FP32 key
-// offset, then the largest error of the FP32 prediction on the leaf's keys
+// offset, the intercept moved to centre the errors, then the largest error
+// of the FP32 prediction on the leaf's keys
 template <typename K>
 __global__ __launch_bounds__(256) void k_leaf_fit(const K *__restrict__ keys,
                                                   const uint64_t *__restrict__ first,
@@ -180,6 +181,33 @@ __global__ __launch_bounds__(256) void k_leaf_fit(const K *__restrict__ keys,
         sm = m;
     }
     __syncthreads();
+    // centre the window: the least-squares line's largest errors above and
+    // below the ranks (P, N) are rarely equal, and moving the intercept by
+    // (P - N) / 2 makes the largest |error| about (P + N) / 2
+    {
+        int64_t up = 0, dn = 0;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            const int64_t d = predict(ukey<K>(keys[i]), sm) - (int64_t)i;
+            up = max(up, d);
+            dn = max(dn, -d);
+        }
+        red[0][threadIdx.x] = (double)up;
+        red[1][threadIdx.x] = (double)dn;
+        __syncthreads();
+        for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+                red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double shift = rint((red[0][0] - red[1][0]) / 2.0);
+            if (shift != 0.0 && fabs(shift) < 16777216.0) sm.icept = (float)((double)sm.icept - shift);
+        }
+        __syncthreads();
+    }
+    // the window's half width: the largest error of the model as stored
     const LeafModel m = sm;
     uint32_t e = 0;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
