@@ -460,6 +460,20 @@ def layout_traffic(layout, n):
     return None if t is None else round(t)
 
 
+def involution_roofline(layout, n, ms):
+    """Algorithmic fraction of one involution-family permute (2 * n * 8 bytes
+    over the call) and, at n = 2^30, its HBM bytes per call from the committed
+    PMC summary with the fraction they make of the peak"""
+    out = {"algorithmic_frac": round(2.0 * n * 8 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "traffic": None, "traffic_frac": None}
+    t = measured_traffic_entry("involutions_n2p30_u64_traffic.txt") if n == 1 << 30 else None
+    if t and f"{layout}_read_GB" in t:
+        traffic = (t[f"{layout}_read_GB"] + t[f"{layout}_write_GB"]) * 1e9
+        out["traffic"] = round(traffic)
+        out["traffic_frac"] = round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
 def other_layouts(args, A, keys, stream, rng, samples=20_000):
     """BST, mvEB and classic vEB permutes (and inverses) of the headline's
     n = 2^log2n uint64 keys in the same buffer: a sampled slot check against
@@ -514,7 +528,7 @@ def other_layouts(args, A, keys, stream, rng, samples=20_000):
         got = A[torch.tensor(pos, dtype=torch.int64, device=A.device)].cpu().numpy()
         ok = bool(np.array_equal(got, want))
         inv[layout] = {"ms": round(ms, 3), "Gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
-                       "layout_sample": ok}
+                       **involution_roofline(layout, n, ms), "layout_sample": ok}
         log(f"[bench] {layout} involutions permute {ms:.3f} ms, sample check {ok}")
     out["involutions"] = inv
     return out
@@ -610,11 +624,18 @@ def run_layout(args, dev, layout):
 def run_involutions(args, dev):
     """The involution family at n = 2^log2n uint64: each permutation `steps`
     times from the sorted keys 0..n-1, its time per call and a full check of
-    the last call's layout against the oracle's position functions"""
+    the last call's layout against the oracle's position functions.  Four
+    copies of 2^27 int64 keys (1 GiB read, 1 GiB written) open the run to
+    scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
     import torch
     import tl_amd
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
+    cal_a = torch.arange(1 << 27, device=dev, dtype=torch.int64)
+    cal_b = torch.empty_like(cal_a)
+    for _ in range(4):
+        cal_b.copy_(cal_a)
+    del cal_a, cal_b
     n = 1 << args.log2n
     A = torch.empty(n, dtype=torch.int64, device=dev)
     out = {}
@@ -630,8 +651,9 @@ def run_involutions(args, dev):
         else:
             pos = [o.or_btree_pos(int(x), n, b) for x in ranks]
         got = A[torch.tensor(pos, dtype=torch.int64, device=dev)].cpu().numpy()
-        out[layout] = {"b": b, "ms": round(float(np.mean(times)), 3),
-                       "runs_ms": [round(t, 3) for t in times],
+        ms = float(np.mean(times))
+        out[layout] = {"b": b, "ms": round(ms, 3), "runs_ms": [round(t, 3) for t in times],
+                       **involution_roofline(layout, n, ms),
                        "layout_sample": bool(np.array_equal(got, ranks))}
         log(f"[bench] {layout} involutions {np.mean(times):.3f} ms")
     print(json.dumps({"workload": f"involution family, n=2^{args.log2n} uint64", "layouts": out}),
