@@ -346,5 +346,7 @@ uint64_t or_involution(uint64_t r, uint64_t x, uint64_t m) {
     int64_t inv, y;
     or_egcd(x / g, m / g, &inv, &y);
     if (inv < 0) inv += (int64_t)(m / g);
-    return g * (r * (uint64_t)inv % (m / g));
+    /* exact where the reference's 64-bit product r * inv would wrap (2^64 and
+       more; no test size reaches it) */
+    return g * (uint64_t)((unsigned __int128)r * (uint64_t)inv % (m / g));
 }

@@ -54,3 +54,37 @@ def test_permute_emulated(emulator, layout, width, sizes):
         assert r.stderr.count(", 0 wrong") == per * len(sizes), r.stderr[-4000:]
     else:
         assert r.stderr.count("0 wrong, inverse 0 wrong") == len(sizes), r.stderr[-4000:]
+
+
+def test_modinv_partner_exact_to_2p40():
+    """The shuffle passes' modular-inverse partner (tl_modinv.hpp, built for
+    the host) against Python's exact arithmetic, x -> g * (r * (x / g)^-1 mod
+    (m / g)) with g = gcd(x, m) and the inverse from pow(., -1, .), at moduli
+    below 2^32 and up to 2^40 (arrays of 2^32 keys and more, which no test
+    permutes), with common factors of x and m and multipliers from 1 to m - 1."""
+    import math
+
+    import numpy as np
+    subprocess.run(["make", "-C", EMU, "_build/emu_modinv"], check=True, capture_output=True)
+    rng = np.random.default_rng(11)
+    cases = []
+    fixed = [2**32 - 1, 2**32, 2**32 + 1, 3 * 2**31, 2**40 - 1, 2**40 - 2, 17**9 - 1]
+    for i in range(4000):
+        m = fixed[i % len(fixed)] if i < 70 else int(
+            rng.integers(2**32, 2**40) if i % 2 else rng.integers(3, 2**32))
+        x = int(rng.integers(1, m))
+        if i % 3 == 0:   # a common factor with m
+            f = math.gcd(m, int(rng.integers(2, 2**12)))
+            x = max(1, (x // f) * f) % m or f
+        r = [17, math.isqrt(m), m - 1, int(rng.integers(1, m))][i % 4]
+        cases.append((x, m, r))
+    inp = "".join(f"{x} {m} {r}\n" for x, m, r in cases)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    out = subprocess.run([os.path.join(EMU, "_build", "emu_modinv")], input=inp, env=env,
+                         capture_output=True, text=True, check=True, timeout=120).stdout.split()
+    assert len(out) == len(cases)
+    for (x, m, r), got in zip(cases, out):
+        g = math.gcd(x, m)
+        mg = m // g
+        want = g * ((r * pow(x // g, -1, mg)) % mg)
+        assert int(got) == want, (x, m, r)

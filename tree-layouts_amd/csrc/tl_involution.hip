@@ -17,6 +17,7 @@
 // round whose partitions fit the LDS budget) run in LDS: one workgroup loads a
 // partition, runs every remaining pass on it there and writes it back once.
 #include "tl_internal.hpp"
+#include "tl_modinv.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -50,12 +51,6 @@ __device__ inline uint64_t fdiv(uint64_t x, const FastDiv &f) {
     return f.d == 1 ? x : x / f.d;
 }
 
-// quotient of a / b by 32-bit division when both fit
-__device__ inline uint64_t udiv(uint64_t a, uint64_t b) {
-    if (((a | b) >> 32) == 0) return (uint32_t)a / (uint32_t)b;
-    return a / b;
-}
-
 // x with its nd lowest base-k digits reversed (the higher digits kept):
 // rev_base_d / rev_base_b of gpu/src/involutions.cu:34-69, digit by digit from
 // the low end instead of from a power computed with pow()
@@ -68,31 +63,6 @@ __device__ inline uint64_t rev_digits(uint64_t x, const FastDiv &k, uint32_t nd)
         p *= k.d;
     }
     return x * p + r;
-}
-
-// the partner of x in [1, m) under the involution x -> g * (r * inv(x / g) mod
-// (m / g)), g = gcd(x, m), inv the inverse modulo m / g: `involution` of
-// gpu/src/involutions.cu:106-113.  One extended-Euclid run gives both g and the
-// inverse (it keeps a = s_a * x mod m for both remainders), where the
-// reference runs gcd and egcd separately.
-__device__ inline uint64_t mod_partner(uint64_t x, uint64_t m, uint64_t r) {
-    uint64_t a = m, b = x;
-    int64_t sa = 0, sb = 1;
-    while (b) {
-        const uint64_t q = udiv(a, b);
-        const uint64_t t = a - q * b;
-        a = b;
-        b = t;
-        const int64_t st = sa - (int64_t)q * sb;
-        sa = sb;
-        sb = st;
-    }
-    // a = g; sa * (x / g) = 1 (mod m / g)
-    const uint64_t mg = udiv(m, a);
-    int64_t inv = sa % (int64_t)mg;
-    if (inv < 0) inv += (int64_t)mg;
-    const uint64_t v = (uint64_t)inv * (r % mg);
-    return a * (v - udiv(v, mg) * mg);
 }
 
 // ---------------------------------------------------------------------------
