@@ -110,8 +110,10 @@ tl_status tl_permute_btree_noalloc(void *dev_A, uint64_t n, uint64_t b, tl_dtype
 /* The reference's second set of permutation programs, in which every pass
  * swaps each key with a closed-form partner (an involution).  Same layouts as
  * tl_permute_{bst,btree,mveb}; more passes and scattered partners, so slower
- * (the gather / cycle-leader path above is the fast one).  No scratch beyond
- * the LDS tables of the mvEB's in-LDS base cases.  Replace
+ * (the gather / cycle-leader path above is the fast one).  n < 2^40.  No
+ * n-sized scratch: the library's context holds the pass list and the partner
+ * tables of pass shapes that repeat over partitions or run in LDS (at most
+ * 2^17 four-byte entries per shape).  Replace
  *   timePermuteBST<T>(dev_A, n)       (gpu/src/bst-involutions.cuh:167),
  *   timePermuteBtree<T>(dev_A, n, b)  (gpu/src/btree-involutions.cuh:151),
  *   timePermutevEB<T>(dev_A, n)       (gpu/src/mveb-involutions.cuh:503).
