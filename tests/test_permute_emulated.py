@@ -61,7 +61,8 @@ def test_modinv_partner_exact_to_2p40():
     the host) against Python's exact arithmetic, x -> g * (r * (x / g)^-1 mod
     (m / g)) with g = gcd(x, m) and the inverse from pow(., -1, .), at moduli
     below 2^32 and up to 2^40 (arrays of 2^32 keys and more, which no test
-    permutes), with common factors of x and m and multipliers from 1 to m - 1."""
+    permutes), with common factors of x and m and multipliers from 1 to m - 1,
+    and every x of the moduli below 200."""
     import math
 
     import numpy as np
@@ -78,6 +79,9 @@ def test_modinv_partner_exact_to_2p40():
             x = max(1, (x // f) * f) % m or f
         r = [17, math.isqrt(m), m - 1, int(rng.integers(1, m))][i % 4]
         cases.append((x, m, r))
+    for m in range(2, 200):   # every x of small moduli, m / g down to 2
+        for x in range(1, m):
+            cases.append((x, m, [1, 17, m - 1][x % 3]))
     inp = "".join(f"{x} {m} {r}\n" for x, m, r in cases)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
     out = subprocess.run([os.path.join(EMU, "_build", "emu_modinv")], input=inp, env=env,

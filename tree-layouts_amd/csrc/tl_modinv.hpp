@@ -32,6 +32,26 @@ __device__ inline uint64_t mulmod40(uint64_t a, uint64_t b, uint64_t m) {
 // egcd separately; the product is taken modulo m / g exactly, where the
 // reference's 64-bit r * inv would wrap once it reaches 2^64.
 __device__ inline uint64_t mod_partner(uint64_t x, uint64_t m, uint64_t r) {
+    if (m < (1ull << 32)) {
+        // all in 32 bits: the coefficients of x are kept modulo 2^32; the one
+        // returned, s with s * x = g (mod m), has |s| <= m / (2g) < 2^31, so
+        // its 32-bit pattern read as signed is exact (only the coefficient of
+        // the last step, +-m / g, can leave that range, and it is dropped)
+        uint32_t a = (uint32_t)m, b = (uint32_t)x, sa = 0, sb = 1;
+        while (b) {
+            const uint32_t q = a / b;
+            const uint32_t t = a - q * b;
+            a = b;
+            b = t;
+            const uint32_t st = sa - q * sb;
+            sa = sb;
+            sb = st;
+        }
+        const uint32_t mg = (uint32_t)m / a;
+        int64_t inv = (int64_t)(int32_t)sa;
+        if (inv < 0) inv += (int64_t)mg;
+        return (uint64_t)a * mulmod40((uint64_t)inv, r % mg, mg);
+    }
     uint64_t a = m, b = x;
     int64_t sa = 0, sb = 1;
     while (b) {
